@@ -1,0 +1,316 @@
+// FDK pre-weighting and row-wise ramp filtering for gfx950.
+//
+// Behaviour follows kernelkit/processing.py of the reference:
+// * preweight (:149-188): p[a, r, c] *= |d - s| / |corner(r, c) - s| with
+//   corner(r, c) = d_min + c * U + r * V (pixel corners, world units);
+// * filter (:96-126): every detector row is zero-padded to N (a power of two,
+//   N >= 2 * cols), transformed, multiplied by the real response H, inverse
+//   transformed, and its first `cols` entries kept.  The reference does this
+//   in float64 with cuFFT, one Python-level loop iteration per angle.
+//
+// MI355X design: one workgroup filters two detector rows at once.  They are
+// packed as the real and imaginary parts of one complex signal; because H
+// is real and even, the real part of the result is the filtered first row
+// and the imaginary part the filtered second row.  The transform runs in
+// LDS: a radix-2 decimation-in-frequency pass (natural -> bit-reversed
+// order), the multiplication by H stored in bit-reversed order with 1/N
+// folded in, and a decimation-in-time inverse pass (bit-reversed ->
+// natural), so no explicit permutation is needed.  Twiddles and H come from
+// float64 host tables rounded once to float32.  The pre-weight can be fused
+// into the load, and the output can be written anywhere through strides
+// (e.g. straight into the backprojector's padded texture).
+#include <cmath>
+#include <vector>
+
+#include "kk_internal.h"
+
+namespace {
+
+struct PreweightAngle {  // world units, float32 after an origin shift
+  float qx, qy, qz;      // d_min - s
+  float ux, uy, uz;      // column step u * pixel_width
+  float vx, vy, vz;      // row step v * pixel_height
+  float central;         // |d - s|
+  float _pad;
+};
+
+__device__ __forceinline__ float preweight_factor(const PreweightAngle& P,
+                                                  int32_t r, int32_t c) {
+  const float fr = static_cast<float>(r), fc = static_cast<float>(c);
+  const float x = fmaf(fr, P.vx, fmaf(fc, P.ux, P.qx));
+  const float y = fmaf(fr, P.vy, fmaf(fc, P.uy, P.qy));
+  const float z = fmaf(fr, P.vz, fmaf(fc, P.uz, P.qz));
+  return P.central / sqrtf(fmaf(x, x, fmaf(y, y, z * z)));
+}
+
+__global__ __launch_bounds__(256) void preweight_kernel(
+    float* __restrict__ p, int64_t s0, int64_t s1, int64_t s2, int32_t rows,
+    int32_t cols, const PreweightAngle* __restrict__ tab) {
+  const int32_t c = blockIdx.x * 256 + threadIdx.x;
+  const int32_t r = blockIdx.y;
+  const int32_t a = blockIdx.z;
+  if (c >= cols) return;
+  const int64_t o = a * s0 + r * s1 + c * s2;
+  p[o] *= preweight_factor(tab[a], r, c);
+}
+
+struct FilterArgs {
+  const float* in;
+  int64_t is0, is1, is2;
+  float* out;
+  int64_t os0, os1, os2;
+  int64_t total_rows;
+  int32_t rows, cols, n, log2n;
+  const PreweightAngle* pw;  // optional fused pre-weight
+  const float2* tw;          // exp(-2 pi i k / N), k < N/2
+  const float* hb;           // H[bitrev(k)] / N
+};
+
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(fmaf(a.x, b.x, -a.y * b.y), fmaf(a.x, b.y, a.y * b.x));
+}
+
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
+}
+
+__global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
+  extern __shared__ float2 buf[];
+  const int32_t N = A.n;
+  const int64_t g0 = 2 * int64_t(blockIdx.x);
+  const int64_t g1 = g0 + 1;
+  const bool has1 = g1 < A.total_rows;
+  const int32_t a0 = static_cast<int32_t>(g0 / A.rows);
+  const int32_t r0 = static_cast<int32_t>(g0 % A.rows);
+  const int32_t a1 = has1 ? static_cast<int32_t>(g1 / A.rows) : a0;
+  const int32_t r1 = has1 ? static_cast<int32_t>(g1 % A.rows) : r0;
+  const float* in0 = A.in + a0 * A.is0 + r0 * A.is1;
+  const float* in1 = A.in + a1 * A.is0 + r1 * A.is1;
+
+  for (int32_t c = threadIdx.x; c < N; c += 256) {
+    float re = 0.f, im = 0.f;
+    if (c < A.cols) {
+      re = in0[c * A.is2];
+      if (has1) im = in1[c * A.is2];
+      if (A.pw) {
+        re *= preweight_factor(A.pw[a0], r0, c);
+        if (has1) im *= preweight_factor(A.pw[a1], r1, c);
+      }
+    }
+    buf[c] = make_float2(re, im);
+  }
+  __syncthreads();
+
+  const int32_t half = N >> 1;
+  // forward transform, decimation in frequency: natural -> bit-reversed
+  for (int32_t m = half, ts = 1; m >= 1; m >>= 1, ts <<= 1) {
+    for (int32_t j = threadIdx.x; j < half; j += 256) {
+      const int32_t k = j & (m - 1);
+      const int32_t i0 = ((j - k) << 1) + k;
+      const int32_t i1 = i0 + m;
+      const float2 x0 = buf[i0], x1 = buf[i1];
+      buf[i0] = make_float2(x0.x + x1.x, x0.y + x1.y);
+      buf[i1] = cmul(make_float2(x0.x - x1.x, x0.y - x1.y), A.tw[k * ts]);
+    }
+    __syncthreads();
+  }
+  // multiply by the real response (bit-reversed order, 1/N folded in)
+  for (int32_t c = threadIdx.x; c < N; c += 256) {
+    const float h = A.hb[c];
+    const float2 x = buf[c];
+    buf[c] = make_float2(x.x * h, x.y * h);
+  }
+  __syncthreads();
+  // inverse transform, decimation in time: bit-reversed -> natural
+  for (int32_t m = 1, ts = half; m < N; m <<= 1, ts >>= 1) {
+    for (int32_t j = threadIdx.x; j < half; j += 256) {
+      const int32_t k = j & (m - 1);
+      const int32_t i0 = ((j - k) << 1) + k;
+      const int32_t i1 = i0 + m;
+      const float2 x0 = buf[i0];
+      const float2 t = cmulc(buf[i1], A.tw[k * ts]);
+      buf[i0] = make_float2(x0.x + t.x, x0.y + t.y);
+      buf[i1] = make_float2(x0.x - t.x, x0.y - t.y);
+    }
+    __syncthreads();
+  }
+  float* out0 = A.out + a0 * A.os0 + r0 * A.os1;
+  float* out1 = A.out + a1 * A.os0 + r1 * A.os1;
+  for (int32_t c = threadIdx.x; c < A.cols; c += 256) {
+    const float2 x = buf[c];
+    out0[c * A.os2] = x.x;
+    if (has1) out1[c * A.os2] = x.y;
+  }
+}
+
+}  // namespace
+
+struct kk_filter_s {
+  int device = 0;
+  int32_t n = 0;
+  float2* tw = nullptr;
+  float* hb = nullptr;
+  PreweightAngle* pw = nullptr;
+  int32_t pw_n = 0, pw_cap = 0;
+};
+
+extern "C" int kk_filter_create(kk_filter** out) {
+  if (out == nullptr) return kk::fail(KK_ERR_INVALID, "kk_filter_create");
+  auto* f = new (std::nothrow) kk_filter();
+  if (f == nullptr) return kk::fail(KK_ERR_INVALID, "out of host memory");
+  hipError_t e = hipGetDevice(&f->device);
+  if (e != hipSuccess) {
+    delete f;
+    return kk::hip_fail(e, "hipGetDevice");
+  }
+  *out = f;
+  return KK_OK;
+}
+
+extern "C" int kk_filter_destroy(kk_filter* f) {
+  if (f == nullptr) return KK_OK;
+  if (f->tw) hipFree(f->tw);
+  if (f->hb) hipFree(f->hb);
+  if (f->pw) hipFree(f->pw);
+  delete f;
+  return KK_OK;
+}
+
+extern "C" int kk_filter_set_response(kk_filter* f, const double* H,
+                                      int32_t n) {
+  if (f == nullptr || H == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_filter_set_response: null");
+  if (n < 2 || (n & (n - 1)) != 0 || n > 8192)
+    return kk::fail(KK_ERR_INVALID,
+                    "filter length must be a power of two in [2, 8192]");
+  int log2n = 0;
+  while ((1 << log2n) < n) ++log2n;
+  std::vector<float2> tw(n / 2);
+  const double pi = 3.14159265358979323846;
+  for (int32_t k = 0; k < n / 2; ++k) {
+    const double ang = -2.0 * pi * double(k) / double(n);
+    tw[k] = make_float2(float(std::cos(ang)), float(std::sin(ang)));
+  }
+  std::vector<float> hb(n);
+  for (int32_t k = 0; k < n; ++k) {
+    int32_t rev = 0;
+    for (int b = 0; b < log2n; ++b) rev |= ((k >> b) & 1) << (log2n - 1 - b);
+    hb[k] = float(H[rev] / double(n));
+  }
+  if (f->tw) hipFree(f->tw);
+  if (f->hb) hipFree(f->hb);
+  f->tw = nullptr;
+  f->hb = nullptr;
+  f->n = 0;
+  hipError_t e = hipMalloc(&f->tw, sizeof(float2) * tw.size());
+  if (e != hipSuccess) return kk::hip_fail(e, "hipMalloc twiddles");
+  e = hipMalloc(&f->hb, sizeof(float) * hb.size());
+  if (e != hipSuccess) return kk::hip_fail(e, "hipMalloc response");
+  e = hipMemcpy(f->tw, tw.data(), sizeof(float2) * tw.size(),
+                hipMemcpyHostToDevice);
+  if (e != hipSuccess) return kk::hip_fail(e, "hipMemcpy twiddles");
+  e = hipMemcpy(f->hb, hb.data(), sizeof(float) * hb.size(),
+                hipMemcpyHostToDevice);
+  if (e != hipSuccess) return kk::hip_fail(e, "hipMemcpy response");
+  f->n = n;
+  return KK_OK;
+}
+
+extern "C" int kk_filter_set_preweight(kk_filter* f, const double* p,
+                                       int32_t n) {
+  if (f == nullptr || p == nullptr || n <= 0)
+    return kk::fail(KK_ERR_INVALID, "kk_filter_set_preweight: bad args");
+  std::vector<PreweightAngle> tab(n);
+  for (int32_t k = 0; k < n; ++k) {
+    const double* q = p + 10 * int64_t(k);
+    PreweightAngle& e = tab[k];
+    e.qx = float(q[0]);
+    e.qy = float(q[1]);
+    e.qz = float(q[2]);
+    e.ux = float(q[3]);
+    e.uy = float(q[4]);
+    e.uz = float(q[5]);
+    e.vx = float(q[6]);
+    e.vy = float(q[7]);
+    e.vz = float(q[8]);
+    e.central = float(q[9]);
+    e._pad = 0.f;
+  }
+  if (n > f->pw_cap) {
+    if (f->pw) hipFree(f->pw);
+    f->pw = nullptr;
+    f->pw_cap = 0;
+    hipError_t e = hipMalloc(&f->pw, sizeof(PreweightAngle) * n);
+    if (e != hipSuccess) return kk::hip_fail(e, "hipMalloc preweight");
+    f->pw_cap = n;
+  }
+  hipError_t e = hipMemcpy(f->pw, tab.data(), sizeof(PreweightAngle) * n,
+                           hipMemcpyHostToDevice);
+  if (e != hipSuccess) return kk::hip_fail(e, "hipMemcpy preweight");
+  f->pw_n = n;
+  return KK_OK;
+}
+
+extern "C" int kk_preweight(const kk_filter* f, float* proj, int32_t n,
+                            int32_t rows, int32_t cols, const int64_t st[3],
+                            void* stream) {
+  if (f == nullptr || proj == nullptr || st == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_preweight: null argument");
+  if (f->pw == nullptr || f->pw_n != n)
+    return kk::fail(KK_ERR_NOT_SET, "kk_preweight: pre-weights not set for "
+                                    "%d angles",
+                    n);
+  if (rows <= 0 || cols <= 0 || rows > 65535 || n > 65535)
+    return kk::fail(KK_ERR_INVALID, "kk_preweight: bad sizes");
+  const dim3 grid((cols + 255) / 256, rows, n);
+  hipLaunchKernelGGL(preweight_kernel, grid, dim3(256), 0,
+                     static_cast<hipStream_t>(stream), proj, st[0], st[1],
+                     st[2], rows, cols, f->pw);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kk::hip_fail(e, "preweight_kernel");
+  return KK_OK;
+}
+
+extern "C" int kk_filter_rows(const kk_filter* f, const float* in,
+                              const int64_t is[3], float* out,
+                              const int64_t os[3], int32_t n, int32_t rows,
+                              int32_t cols, int32_t fuse_preweight,
+                              void* stream) {
+  if (f == nullptr || in == nullptr || out == nullptr || is == nullptr ||
+      os == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_filter_rows: null argument");
+  if (f->n == 0) return kk::fail(KK_ERR_NOT_SET, "filter response not set");
+  if (n <= 0 || rows <= 0 || cols <= 0)
+    return kk::fail(KK_ERR_INVALID, "kk_filter_rows: bad sizes");
+  if (cols > f->n)
+    return kk::fail(KK_ERR_INVALID,
+                    "kk_filter_rows: %d columns exceed the filter length %d",
+                    cols, f->n);
+  if (fuse_preweight && (f->pw == nullptr || f->pw_n != n))
+    return kk::fail(KK_ERR_NOT_SET, "kk_filter_rows: pre-weights not set");
+  FilterArgs A{};
+  A.in = in;
+  A.is0 = is[0];
+  A.is1 = is[1];
+  A.is2 = is[2];
+  A.out = out;
+  A.os0 = os[0];
+  A.os1 = os[1];
+  A.os2 = os[2];
+  A.total_rows = int64_t(n) * rows;
+  A.rows = rows;
+  A.cols = cols;
+  A.n = f->n;
+  A.pw = fuse_preweight ? f->pw : nullptr;
+  A.tw = f->tw;
+  A.hb = f->hb;
+  const int64_t pairs = (A.total_rows + 1) / 2;
+  if (pairs >= (int64_t(1) << 31))
+    return kk::fail(KK_ERR_UNSUPPORTED, "kk_filter_rows: too many rows");
+  const size_t lds = sizeof(float2) * size_t(f->n);
+  hipLaunchKernelGGL(filter_rows_kernel, dim3(unsigned(pairs)), dim3(256),
+                     lds, static_cast<hipStream_t>(stream), A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kk::hip_fail(e, "filter_rows_kernel");
+  return KK_OK;
+}

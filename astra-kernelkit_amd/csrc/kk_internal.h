@@ -1,0 +1,92 @@
+// Internal declarations shared by the HIP translation units of the
+// cone-beam library (gfx950).  Nothing here is part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernelkit_hip.h"
+
+namespace kk {
+
+// Record an error message for kk_last_error() and return `code`.
+int fail(int code, const char* fmt, ...);
+// Record a HIP failure and return KK_ERR_HIP.
+int hip_fail(hipError_t e, const char* what);
+
+constexpr int kHaloLo = KK_HALO_LO;
+constexpr int kHaloHi = KK_HALO_HI;
+constexpr int kHalo = KK_HALO_LO + KK_HALO_HI;
+
+// Number of XCDs on MI355X; consecutive workgroups are dealt round-robin
+// over them, each XCD with its own L2.
+constexpr int kNumXcd = 8;
+
+// Per-angle forward-projection record, float32, in the ray frame (a, b, c)
+// of that angle: a = ray (major) axis, b/c = the two in-plane axes, with
+// (a, b, c) = (x, y, z), (y, x, z) or (z, y, x) for a = x, y, z.
+struct alignas(16) FpAngle {
+  float qa, qb, qc, _p0;    // centre of pixel (0,0) minus source
+  float ua, ub, uc, _p1;    // column step U
+  float va, vb, vc, _p2;    // row step V
+  float sig, taub, tauc, _p3;  // slice i <-> b-coordinate offsets (below)
+  float sc1, sc2, oscale, _p4; // sqrt(1 + a1^2 sc1 + a2^2 sc2) * oscale
+  int32_t na, nb, nc, pair;    // extents along a, b, c; pair: c contiguous
+  int32_t sa, sb, sc, axis;    // texture strides along a, b, c; world axis a
+  int32_t _p5[4];
+};
+// Ray through pixel (r, c): D = q + c*U + r*V (relative to the source),
+// a1 = D_b / D_a, a2 = D_c / D_a.  Array coordinate along b at slice i:
+// x_b(i) = a1 * (i + sig) + taub, with sig = 1/2 - n_a/2 - s_a and
+// taub = s_b + n_b/2 - 1/2 (both computed in float64 from the
+// volume-centred geometry, then rounded once).
+static_assert(sizeof(FpAngle) == 128, "FpAngle layout");
+
+// Per-angle backprojection record: affine forms of the voxel centre X
+// (voxel frame, world order).  u = nu(X)/den(X) and v = nv(X)/den(X) are
+// array coordinates on the detector (the texel-centre 1/2 already removed),
+// 1/den^2 is the adjoint weight.  fast != 0 when nu and den do not depend
+// on z (circular orbit about z): then u and 1/den are shared by a z-run.
+struct alignas(16) BpAngle {
+  float nux, nuy, nuz, nuw;
+  float nvx, nvy, nvz, nvw;
+  float dx, dy, dz, dw;
+  int32_t fast, _p0, _p1, _p2;
+};
+static_assert(sizeof(BpAngle) == 64, "BpAngle layout");
+
+}  // namespace kk
+
+struct kk_plan_s {
+  int device = 0;
+  // forward projection table
+  kk::FpAngle* fp = nullptr;
+  int32_t fp_n = 0, fp_cap = 0;
+  int32_t fp_shape[3] = {0, 0, 0};
+  bool fp_set = false;
+  // backprojection table
+  kk::BpAngle* bp = nullptr;
+  int32_t bp_n = 0, bp_cap = 0;
+  bool bp_set = false;
+};
+
+// Map a hardware workgroup id to a logical tile id such that, inside every
+// chunk of kNumXcd * group consecutive ids, each XCD receives `group`
+// consecutive logical tiles.  Tail chunks keep the identity mapping.
+__device__ __forceinline__ int64_t kk_xcd_remap(int64_t bid, int64_t total,
+                                                int32_t group) {
+  if (group <= 1) return bid;
+  const int64_t span = (int64_t)kk::kNumXcd * group;
+  const int64_t chunk = bid / span;
+  if ((chunk + 1) * span > total) return bid;
+  const int64_t r = bid - chunk * span;
+  return chunk * span + (r % kk::kNumXcd) * group + r / kk::kNumXcd;
+}
+
+__device__ __forceinline__ float kk_clamp(float x, float lo, float hi) {
+  return __builtin_amdgcn_fmed3f(x, lo, hi);
+}
+
+// Two consecutive floats, 4-byte aligned: one global_load_dwordx2.
+typedef float kk_float2u __attribute__((ext_vector_type(2), aligned(4)));

@@ -1,0 +1,339 @@
+// Host side of the C ABI: error reporting, plans and the per-angle device
+// tables derived (in float64) from the reference's kernel parameters.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kk_internal.h"
+
+namespace {
+thread_local std::string g_last_error;
+}
+
+namespace kk {
+
+int fail(int code, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char* what) {
+  return fail(KK_ERR_HIP, "HIP error %d (%s) in %s", static_cast<int>(e),
+              hipGetErrorString(e), what);
+}
+
+}  // namespace kk
+
+#define KK_HIP(call)                                          \
+  do {                                                        \
+    hipError_t e_ = (call);                                   \
+    if (e_ != hipSuccess) return kk::hip_fail(e_, #call);     \
+  } while (0)
+
+extern "C" const char* kk_last_error(void) { return g_last_error.c_str(); }
+
+extern "C" const char* kk_version(void) { return "kernelkit-hip 0.1 gfx950"; }
+
+extern "C" int kk_plan_create(kk_plan** out) {
+  if (out == nullptr) return kk::fail(KK_ERR_INVALID, "kk_plan_create: null");
+  auto* p = new (std::nothrow) kk_plan();
+  if (p == nullptr) return kk::fail(KK_ERR_INVALID, "out of host memory");
+  hipError_t e = hipGetDevice(&p->device);
+  if (e != hipSuccess) {
+    delete p;
+    return kk::hip_fail(e, "hipGetDevice");
+  }
+  *out = p;
+  return KK_OK;
+}
+
+extern "C" int kk_plan_destroy(kk_plan* plan) {
+  if (plan == nullptr) return KK_OK;
+  int dev = 0;
+  hipGetDevice(&dev);
+  hipSetDevice(plan->device);
+  if (plan->fp) hipFree(plan->fp);
+  if (plan->bp) hipFree(plan->bp);
+  hipSetDevice(dev);
+  delete plan;
+  return KK_OK;
+}
+
+namespace {
+
+template <typename T>
+int upload_table(T** dev_ptr, int32_t* cap, const std::vector<T>& host) {
+  const int32_t n = static_cast<int32_t>(host.size());
+  if (n > *cap) {
+    if (*dev_ptr) KK_HIP(hipFree(*dev_ptr));
+    *dev_ptr = nullptr;
+    *cap = 0;
+    KK_HIP(hipMalloc(reinterpret_cast<void**>(dev_ptr), sizeof(T) * n));
+    *cap = n;
+  }
+  KK_HIP(hipMemcpy(*dev_ptr, host.data(), sizeof(T) * n,
+                   hipMemcpyHostToDevice));
+  return KK_OK;
+}
+
+}  // namespace
+
+// Host-only construction of the forward-projection table.  `axes`
+// (nullable) gives the per-angle ray axis computed by the caller from the
+// detector centres; when null the axis is taken from |source - detector
+// centre| with the centre rebuilt from the corner, which needs rows/cols.
+static int build_fp_table(const double* params, int32_t n,
+                          const int32_t* axes, int32_t rows, int32_t cols,
+                          const int32_t shape[3], const double voxel_size[3],
+                          const int32_t volume_axes[3],
+                          std::vector<kk::FpAngle>& tab) {
+  if (params == nullptr || shape == nullptr || voxel_size == nullptr ||
+      volume_axes == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: null argument");
+  {
+    int seen = 0;
+    for (int k = 0; k < 3; ++k)
+      if (volume_axes[k] >= 0 && volume_axes[k] < 3)
+        seen |= 1 << volume_axes[k];
+    if (seen != 7)
+      return kk::fail(KK_ERR_INVALID,
+                      "volume_axes must be a permutation of (0, 1, 2)");
+  }
+  if (n <= 0) return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: no angles");
+  for (int k = 0; k < 3; ++k) {
+    if (shape[k] <= 0)
+      return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: bad volume shape");
+    if (!(voxel_size[k] > 0))
+      return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: bad voxel size");
+  }
+  // texture = padded array in array axis order; stride of each world axis
+  int64_t text[3];
+  for (int k = 0; k < 3; ++k) text[k] = shape[volume_axes[k]] + kk::kHalo;
+  if (text[0] * text[1] * text[2] >= (int64_t(1) << 31))
+    return kk::fail(KK_ERR_UNSUPPORTED,
+                    "volume too large for 32-bit texture indexing");
+  const int64_t astride[3] = {text[1] * text[2], text[2], 1};
+  int32_t tstride[3];
+  for (int k = 0; k < 3; ++k) tstride[volume_axes[k]] = int32_t(astride[k]);
+  if (axes == nullptr && (rows <= 0 || cols <= 0))
+    return kk::fail(KK_ERR_INVALID,
+                    "kk_set_params_fp: rows/cols needed to find ray axes");
+
+  // Output scale table (kernels/cone_fp.py:287-300), reproduced verbatim.
+  const double* v = voxel_size;
+  double osc[3][3];
+  const double eps = 1e-4;
+  if (std::fabs(v[0] / v[1] - 1) > eps || std::fabs(v[0] / v[2] - 1) > eps) {
+    const double t[3][3] = {
+        {(v[1] / v[0]) * (v[1] / v[0]), (v[2] / v[0]) * (v[2] / v[0]),
+         v[0] * v[0]},
+        {(v[0] / v[1]) * (v[0] / v[1]), (v[2] / v[1]) * (v[2] / v[1]),
+         v[0] * v[1]},
+        {(v[0] / v[2]) * (v[0] / v[2]), (v[1] / v[2]) * (v[1] / v[2]),
+         v[0] * v[2]}};
+    std::memcpy(osc, t, sizeof(t));
+  } else {
+    for (int a = 0; a < 3; ++a) {
+      osc[a][0] = 1.0;
+      osc[a][1] = 1.0;
+      osc[a][2] = v[0];
+    }
+  }
+  static const int frame[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}};
+
+  tab.assign(n, kk::FpAngle{});
+  for (int32_t k = 0; k < n; ++k) {
+    double s[3], d0[3], U[3], V[3];
+    for (int t = 0; t < 3; ++t) {
+      s[t] = params[(0 + t) * n + k];
+      d0[t] = params[(3 + t) * n + k];
+      U[t] = params[(6 + t) * n + k];
+      V[t] = params[(9 + t) * n + k];
+    }
+    int m;
+    if (axes) {
+      m = axes[k];
+      if (m < 0 || m > 2)
+        return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: bad ray axis");
+    } else {
+      double best = -1.0;
+      m = 0;
+      for (int t = 0; t < 3; ++t) {
+        const double dc = d0[t] + U[t] * (cols / 2.0) + V[t] * (rows / 2.0);
+        const double dist = std::fabs(s[t] - dc);
+        if (dist > best) {
+          best = dist;
+          m = t;
+        }
+      }
+    }
+    const int a = frame[m][0];
+    int b = frame[m][1], c = frame[m][2];
+    // bilinear interpolation is symmetric in (b, c): make c the contiguous
+    // axis when one of them is, so its two taps are one 8-byte load
+    double sc1 = osc[m][0], sc2 = osc[m][1];
+    if (tstride[c] != 1 && tstride[b] == 1) {
+      const int tmp = b;
+      b = c;
+      c = tmp;
+      const double ts = sc1;
+      sc1 = sc2;
+      sc2 = ts;
+    }
+    double q[3];
+    for (int t = 0; t < 3; ++t) q[t] = d0[t] + 0.5 * U[t] + 0.5 * V[t] - s[t];
+    kk::FpAngle& e = tab[k];
+    std::memset(&e, 0, sizeof(e));
+    e.qa = float(q[a]);
+    e.qb = float(q[b]);
+    e.qc = float(q[c]);
+    e.ua = float(U[a]);
+    e.ub = float(U[b]);
+    e.uc = float(U[c]);
+    e.va = float(V[a]);
+    e.vb = float(V[b]);
+    e.vc = float(V[c]);
+    e.sig = float(0.5 - 0.5 * shape[a] - s[a]);
+    e.taub = float(s[b] + 0.5 * shape[b] - 0.5);
+    e.tauc = float(s[c] + 0.5 * shape[c] - 0.5);
+    e.sc1 = float(sc1);
+    e.sc2 = float(sc2);
+    e.oscale = float(osc[m][2]);
+    e.na = shape[a];
+    e.nb = shape[b];
+    e.nc = shape[c];
+    e.sa = tstride[a];
+    e.sb = tstride[b];
+    e.sc = tstride[c];
+    e.pair = (e.sc == 1) ? 1 : 0;
+    e.axis = m;
+  }
+  return KK_OK;
+}
+
+extern "C" int kk_set_params_fp(kk_plan* plan, const double* params,
+                                int32_t n, const int32_t* axes, int32_t rows,
+                                int32_t cols, const int32_t shape[3],
+                                const double voxel_size[3],
+                                const int32_t volume_axes[3]) {
+  if (plan == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: null plan");
+  std::vector<kk::FpAngle> tab;
+  const int brc = build_fp_table(params, n, axes, rows, cols, shape,
+                                 voxel_size, volume_axes, tab);
+  if (brc != KK_OK) return brc;
+  int dev = 0;
+  KK_HIP(hipGetDevice(&dev));
+  KK_HIP(hipSetDevice(plan->device));
+  int rc = upload_table(&plan->fp, &plan->fp_cap, tab);
+  hipSetDevice(dev);
+  if (rc != KK_OK) return rc;
+  plan->fp_n = n;
+  for (int t = 0; t < 3; ++t) plan->fp_shape[t] = shape[t];
+  plan->fp_set = true;
+  return KK_OK;
+}
+
+static int build_bp_table(const double* params, int32_t n,
+                          std::vector<kk::BpAngle>& tab) {
+  if (params == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_set_params_bp: null argument");
+  if (n <= 0) return kk::fail(KK_ERR_INVALID, "kk_set_params_bp: no angles");
+  tab.assign(n, kk::BpAngle{});
+  for (int32_t k = 0; k < n; ++k) {
+    const double* p = params + 12 * int64_t(k);
+    // numerators minus den/2: texture coordinate -> array coordinate
+    double nu[4], nv[4], de[4];
+    for (int t = 0; t < 4; ++t) {
+      de[t] = p[8 + t];
+      nu[t] = p[0 + t] - 0.5 * de[t];
+      nv[t] = p[4 + t] - 0.5 * de[t];
+    }
+    for (int t = 0; t < 12; ++t)
+      if (!std::isfinite(p[t]))
+        return kk::fail(KK_ERR_INVALID,
+                        "kk_set_params_bp: non-finite parameter at angle %d",
+                        k);
+    kk::BpAngle& e = tab[k];
+    std::memset(&e, 0, sizeof(e));
+    e.nux = float(nu[0]);
+    e.nuy = float(nu[1]);
+    e.nuz = float(nu[2]);
+    e.nuw = float(nu[3]);
+    e.nvx = float(nv[0]);
+    e.nvy = float(nv[1]);
+    e.nvz = float(nv[2]);
+    e.nvw = float(nv[3]);
+    e.dx = float(de[0]);
+    e.dy = float(de[1]);
+    e.dz = float(de[2]);
+    e.dw = float(de[3]);
+    // z-invariant column coordinate and weight: negligible (relative 1e-13)
+    // z coefficients are treated as zero.
+    const double du = std::fabs(nu[0]) + std::fabs(nu[1]) + std::fabs(nu[3]);
+    const double dd = std::fabs(de[0]) + std::fabs(de[1]) + std::fabs(de[3]);
+    e.fast = (std::fabs(nu[2]) <= 1e-13 * du && std::fabs(de[2]) <= 1e-13 * dd)
+                 ? 1
+                 : 0;
+    if (e.fast) {
+      e.nuz = 0.f;
+      e.dz = 0.f;
+    }
+  }
+  return KK_OK;
+}
+
+extern "C" int kk_set_params_bp(kk_plan* plan, const double* params,
+                                int32_t n) {
+  if (plan == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_set_params_bp: null plan");
+  std::vector<kk::BpAngle> tab;
+  const int brc = build_bp_table(params, n, tab);
+  if (brc != KK_OK) return brc;
+  int dev = 0;
+  KK_HIP(hipGetDevice(&dev));
+  KK_HIP(hipSetDevice(plan->device));
+  int rc = upload_table(&plan->bp, &plan->bp_cap, tab);
+  hipSetDevice(dev);
+  if (rc != KK_OK) return rc;
+  plan->bp_n = n;
+  plan->bp_set = true;
+  return KK_OK;
+}
+
+extern "C" int64_t kk_texture_elems(const int64_t s[3]) {
+  if (s == nullptr || s[0] <= 0 || s[1] <= 0 || s[2] <= 0) return -1;
+  return (s[0] + kk::kHalo) * (s[1] + kk::kHalo) * (s[2] + kk::kHalo);
+}
+
+extern "C" int kk_fp_table(const double* params, int32_t n,
+                           const int32_t* axes, int32_t rows, int32_t cols,
+                           const int32_t shape[3], const double voxel_size[3],
+                           const int32_t volume_axes[3], void* out) {
+  if (out == nullptr) return kk::fail(KK_ERR_INVALID, "kk_fp_table: null");
+  std::vector<kk::FpAngle> tab;
+  const int rc = build_fp_table(params, n, axes, rows, cols, shape,
+                                voxel_size, volume_axes, tab);
+  if (rc != KK_OK) return rc;
+  std::memcpy(out, tab.data(), sizeof(kk::FpAngle) * tab.size());
+  return KK_OK;
+}
+
+extern "C" int kk_bp_table(const double* params, int32_t n, void* out) {
+  if (out == nullptr) return kk::fail(KK_ERR_INVALID, "kk_bp_table: null");
+  std::vector<kk::BpAngle> tab;
+  const int rc = build_bp_table(params, n, tab);
+  if (rc != KK_OK) return rc;
+  std::memcpy(out, tab.data(), sizeof(kk::BpAngle) * tab.size());
+  return KK_OK;
+}

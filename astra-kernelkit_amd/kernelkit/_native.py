@@ -1,0 +1,263 @@
+"""ctypes binding of the gfx950 cone-beam library (``include/kernelkit_hip.h``).
+
+This is the thin C-ABI layer that replaces the reference's NVRTC/CuPy kernel
+runtime (``kernelkit/kernel.py:188-226``).  The library is built ahead of
+time by ``build_native.py``; if it is missing, every device operation raises
+:class:`NativeLibraryError` -- there is no CPU fallback.
+
+Device memory and streams come from PyTorch: arrays are ``torch.Tensor`` on
+the HIP device, streams are ``torch.cuda`` streams (their raw ``hipStream_t``
+is passed through).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_lib" / "libkernelkit_hip.so"
+
+KK_OK = 0
+KK_ERR_INVALID = -1
+KK_ERR_HIP = -2
+KK_ERR_NOT_SET = -3
+KK_ERR_SHAPE = -4
+KK_ERR_UNSUPPORTED = -5
+HALO_LO = 1
+HALO_HI = 2
+HALO = HALO_LO + HALO_HI
+
+
+class NativeLibraryError(RuntimeError):
+    """The HIP library is not built or cannot be loaded."""
+
+
+class KernelError(RuntimeError):
+    """A HIP runtime failure reported by the library."""
+
+
+_lib = None
+
+_c_i32p = ctypes.POINTER(ctypes.c_int32)
+_c_i64p = ctypes.POINTER(ctypes.c_int64)
+_c_f64p = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+# name -> (restype, argtypes)
+_SIGNATURES = {
+    "kk_last_error": (ctypes.c_char_p, []),
+    "kk_version": (ctypes.c_char_p, []),
+    "kk_plan_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "kk_plan_destroy": (ctypes.c_int, [_vp]),
+    "kk_set_params_fp": (ctypes.c_int, [_vp, _c_f64p, ctypes.c_int32, _c_i32p,
+                                        ctypes.c_int32, ctypes.c_int32,
+                                        _c_i32p, _c_f64p, _c_i32p]),
+    "kk_set_params_bp": (ctypes.c_int, [_vp, _c_f64p, ctypes.c_int32]),
+    "kk_texture_elems": (ctypes.c_int64, [_c_i64p]),
+    "kk_array_to_texture": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _vp, _vp]),
+    "kk_fill": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_float, _vp]),
+    "kk_cone_fp": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_int32, _vp]),
+    "kk_cone_bp": (ctypes.c_int, [_vp, _vp, _c_i32p, ctypes.c_int32,
+                                  ctypes.c_int32, ctypes.c_int32, _vp,
+                                  _c_i32p, _c_i64p, ctypes.c_float,
+                                  ctypes.c_int32, _vp]),
+    "kk_sirt_fp_residual": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, _vp,
+                                           _c_i64p, _vp, _c_i64p,
+                                           ctypes.c_int32, ctypes.c_int32,
+                                           _vp]),
+    "kk_sirt_bp_update": (ctypes.c_int, [_vp, _vp, _c_i32p, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_int32, _vp,
+                                         _c_i32p, _c_i64p, _vp, _c_i64p, _vp,
+                                         _c_i64p, ctypes.c_float,
+                                         ctypes.c_float, ctypes.c_int32,
+                                         ctypes.c_int32, _vp, _c_i64p,
+                                         ctypes.c_float, _vp]),
+    "kk_recip_or_zero": (ctypes.c_int, [_vp, ctypes.c_int64, _vp]),
+    "kk_fp_table": (ctypes.c_int, [_c_f64p, ctypes.c_int32, _c_i32p,
+                                   ctypes.c_int32, ctypes.c_int32, _c_i32p,
+                                   _c_f64p, _c_i32p, _vp]),
+    "kk_bp_table": (ctypes.c_int, [_c_f64p, ctypes.c_int32, _vp]),
+    "kk_filter_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "kk_filter_destroy": (ctypes.c_int, [_vp]),
+    "kk_filter_set_response": (ctypes.c_int, [_vp, _c_f64p, ctypes.c_int32]),
+    "kk_filter_set_preweight": (ctypes.c_int, [_vp, _c_f64p,
+                                               ctypes.c_int32]),
+    "kk_preweight": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
+                                    ctypes.c_int32, _c_i64p, _vp]),
+    "kk_filter_rows": (ctypes.c_int, [_vp, _vp, _c_i64p, _vp, _c_i64p,
+                                      ctypes.c_int32, ctypes.c_int32,
+                                      ctypes.c_int32, ctypes.c_int32, _vp]),
+}
+
+
+def exported_symbols() -> list:
+    """C symbols this binding requires from the library."""
+    return sorted(_SIGNATURES)
+
+
+def lib():
+    """Load (once) and return the library; raises if it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    path = Path(os.environ.get("KK_LIBRARY", LIB_PATH))
+    if not path.exists():
+        raise NativeLibraryError(
+            f"HIP library not found at {path}; build it with "
+            "`python astra-kernelkit_amd/build_native.py` (hipcc, gfx950).")
+    try:
+        handle = ctypes.CDLL(str(path))
+    except OSError as exc:
+        raise NativeLibraryError(f"cannot load {path}: {exc}") from exc
+    for name, (res, args) in _SIGNATURES.items():
+        fn = getattr(handle, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = handle
+    return _lib
+
+
+def last_error() -> str:
+    return lib().kk_last_error().decode(errors="replace")
+
+
+def check(status: int, exc_map: dict | None = None) -> None:
+    """Raise the Python exception matching a library status code."""
+    if status == KK_OK:
+        return
+    msg = last_error()
+    if exc_map and status in exc_map:
+        raise exc_map[status](msg)
+    if status in (KK_ERR_INVALID, KK_ERR_SHAPE):
+        raise ValueError(msg)
+    if status == KK_ERR_UNSUPPORTED:
+        raise NotImplementedError(msg)
+    if status == KK_ERR_NOT_SET:
+        from kernelkit.kernel import KernelMemoryUnpreparedException
+        raise KernelMemoryUnpreparedException(msg)
+    raise KernelError(msg)
+
+
+def i32(values) -> ctypes.Array:
+    vals = [int(v) for v in values]
+    return (ctypes.c_int32 * len(vals))(*vals)
+
+
+def i64(values) -> ctypes.Array:
+    vals = [int(v) for v in values]
+    return (ctypes.c_int64 * len(vals))(*vals)
+
+
+def f64(values) -> ctypes.Array:
+    vals = [float(v) for v in values]
+    return (ctypes.c_double * len(vals))(*vals)
+
+
+def f64_ptr(arr: np.ndarray):
+    """Pointer to a C-contiguous float64 numpy array (caller keeps it alive)."""
+    assert arr.dtype == np.float64 and arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(_c_f64p)
+
+
+def i32_ptr(arr: np.ndarray):
+    assert arr.dtype == np.int32 and arr.flags["C_CONTIGUOUS"]
+    return arr.ctypes.data_as(_c_i32p)
+
+
+def stream_ptr(stream=None) -> ctypes.c_void_p:
+    """Raw hipStream_t of a torch stream (default: the current stream)."""
+    import torch
+    if stream is None:
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(int(stream.cuda_stream))
+
+
+class Plan:
+    """Owner of a library plan (per-angle device tables)."""
+
+    def __init__(self):
+        ptr = _vp()
+        check(lib().kk_plan_create(ctypes.byref(ptr)))
+        self._ptr = ptr
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._ptr
+
+    def __del__(self):
+        ptr = getattr(self, "_ptr", None)
+        if ptr is not None and ptr.value and _lib is not None:
+            try:
+                _lib.kk_plan_destroy(ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self._ptr = None
+
+
+class Filter:
+    """Owner of a library filter (twiddles, response, pre-weight tables)."""
+
+    def __init__(self):
+        ptr = _vp()
+        check(lib().kk_filter_create(ctypes.byref(ptr)))
+        self._ptr = ptr
+        self.length = 0
+        self.name = None
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._ptr
+
+    def __del__(self):
+        ptr = getattr(self, "_ptr", None)
+        if ptr is not None and ptr.value and _lib is not None:
+            try:
+                _lib.kk_filter_destroy(ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self._ptr = None
+
+
+# numpy views of the device records (csrc/kk_internal.h)
+FP_ANGLE_DTYPE = np.dtype([
+    ("qa", "<f4"), ("qb", "<f4"), ("qc", "<f4"), ("_p0", "<f4"),
+    ("ua", "<f4"), ("ub", "<f4"), ("uc", "<f4"), ("_p1", "<f4"),
+    ("va", "<f4"), ("vb", "<f4"), ("vc", "<f4"), ("_p2", "<f4"),
+    ("sig", "<f4"), ("taub", "<f4"), ("tauc", "<f4"), ("_p3", "<f4"),
+    ("sc1", "<f4"), ("sc2", "<f4"), ("oscale", "<f4"), ("_p4", "<f4"),
+    ("na", "<i4"), ("nb", "<i4"), ("nc", "<i4"), ("pair", "<i4"),
+    ("sa", "<i4"), ("sb", "<i4"), ("sc", "<i4"), ("axis", "<i4"),
+    ("_p5", "<i4", (4,))])
+BP_ANGLE_DTYPE = np.dtype([
+    ("nux", "<f4"), ("nuy", "<f4"), ("nuz", "<f4"), ("nuw", "<f4"),
+    ("nvx", "<f4"), ("nvy", "<f4"), ("nvz", "<f4"), ("nvw", "<f4"),
+    ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("dw", "<f4"),
+    ("fast", "<i4"), ("_p0", "<i4", (3,))])
+assert FP_ANGLE_DTYPE.itemsize == 128 and BP_ANGLE_DTYPE.itemsize == 64
+
+
+def fp_table(params12: np.ndarray, ray_axes, rows, cols, shape, voxel_size,
+             volume_axes) -> np.ndarray:
+    """Host copy of the forward-projection device table."""
+    params12 = np.ascontiguousarray(params12, dtype=np.float64)
+    n = params12.shape[1]
+    out = np.zeros(n, dtype=FP_ANGLE_DTYPE)
+    axes = (None if ray_axes is None
+            else np.ascontiguousarray(ray_axes, dtype=np.int32))
+    check(lib().kk_fp_table(f64_ptr(params12), n,
+                            None if axes is None else i32_ptr(axes), rows,
+                            cols, i32(shape), f64(voxel_size),
+                            i32(volume_axes), out.ctypes.data_as(_vp)))
+    return out
+
+
+def bp_table(params: np.ndarray) -> np.ndarray:
+    """Host copy of the backprojection device table."""
+    params = np.ascontiguousarray(np.asarray(params, dtype=np.float64).ravel())
+    n = params.size // 12
+    out = np.zeros(n, dtype=BP_ANGLE_DTYPE)
+    check(lib().kk_bp_table(f64_ptr(params), n, out.ctypes.data_as(_vp)))
+    return out

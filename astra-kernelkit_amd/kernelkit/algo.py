@@ -1,0 +1,252 @@
+"""Reconstruction algorithms: ``fp``, ``bp``, ``fbp``, ``fdk``, ``sirt``.
+
+Drop-in for ``kernelkit/algo.py`` of the reference (:18-368).  Host inputs
+are copied to the device; results come back as host numpy arrays unless the
+caller passes device outputs (``out``) or asks for them (``return_gpu``).
+
+``fdk`` fuses pre-weight + row filter into one kernel that writes straight
+into the backprojector's texture.  ``sirt`` runs two launches per
+iteration: the forward projection writes ``R * (A x - y)`` into the
+backprojection texture, and the backprojection's epilogue applies
+``x <- clip(mask * (x - C * A^T r))`` and refreshes the forward projector's
+texture of ``x``.
+"""
+from __future__ import annotations
+
+from typing import Any, Callable, Sequence
+
+import numpy as np
+
+from kernelkit import _native
+from kernelkit.geom.proj import Beam, ProjectionGeometry
+from kernelkit.geom.vol import VolumeGeometry
+from kernelkit.kernel import (BaseKernel, Texture, as_device_array,
+                              default_device, is_device_array)
+from kernelkit.kernels.cone_bp import VoxelDrivenConeBP, _world_strides
+from kernelkit.operator import XrayTransform
+from kernelkit.processing import filter_into
+from kernelkit.projector import BackProjector, ForwardProjector
+
+
+def _to_host(t, out=None):
+    host = t.detach().cpu().numpy()
+    if out is not None:
+        out[...] = host
+        return out
+    return host
+
+
+def fp(volume: Any, projection_geometry, volume_geometry: VolumeGeometry,
+       out=None):
+    """Forward projection (``algo.py:18-49``).
+
+    Returns a host array unless ``out`` is a device tensor (then it is
+    filled and returned).
+    """
+    op = XrayTransform(projection_geometry, volume_geometry)
+    vol = as_device_array(volume)
+    dev_out = out if is_device_array(out) else None
+    result = op(vol, out=dev_out)
+    if dev_out is not None:
+        return dev_out
+    return _to_host(result, out if isinstance(out, np.ndarray) else None)
+
+
+def bp(projections: Any, projection_geometry, volume_geometry: VolumeGeometry,
+       filter: Any = None, preproc_fn: Callable = None,
+       kernel: BaseKernel = None, out=None,
+       projection_axes: Sequence[int] = (0, 1, 2), **kwargs):
+    """Backprojection with optional FDK filtering (``algo.py:52-126``).
+
+    ``preproc_fn(projections, projection_geometry)`` runs on the device
+    tensor first; with ``filter`` the projections are pre-weighted and
+    row-filtered (fused, into the backprojector's texture).  Returns a host
+    array unless ``out`` is a device tensor.
+    """
+    import torch
+    projection_axes = tuple(projection_axes)
+    if is_device_array(out):
+        out_dev = out
+    else:
+        out_dev = torch.zeros(tuple(int(s) for s in volume_geometry.shape),
+                              dtype=torch.float32, device=default_device())
+        if isinstance(out, np.ndarray):
+            out_dev.copy_(torch.from_numpy(np.asarray(out, np.float32)))
+    proj = as_device_array(projections)
+    if preproc_fn is not None:
+        if proj is projections:
+            proj = proj.clone()
+        preproc_fn(proj, projection_geometry)
+    kwargs.pop("beam", None)
+    kwargs.update(projection_axes=projection_axes)
+    K = VoxelDrivenConeBP(**kwargs) if kernel is None else kernel
+    ptor = BackProjector(K, texture_type="layered")
+    ptor.projection_geometry = projection_geometry
+    ptor.volume_geometry = volume_geometry
+    if filter is not None:
+        if projection_axes != (0, 1, 2):
+            raise NotImplementedError()
+        tex = Texture(proj.shape, device=proj.device, layered=True)
+        _native.check(_native.lib().kk_fill(
+            _native._vp(tex.ptr), tex.data.numel(), 0.0,
+            _native.stream_ptr()))
+        filter_into(proj, tex.interior(), geoms=list(projection_geometry),
+                    filter=filter)
+        ptor._textures = tex
+    else:
+        ptor.projections = proj
+    ptor.volume = out_dev
+    ptor(additive=True)
+    if is_device_array(out):
+        return _to_host(out_dev)
+    return _to_host(out_dev, out if isinstance(out, np.ndarray) else None)
+
+
+def fbp(projections: Any, projection_geometry, volume_geometry,
+        filter: Any = "ramlak", **bp_kwargs):
+    """Parallel-beam FBP: not implemented, as in the reference
+    (``algo.py:129-166``)."""
+    raise NotImplementedError(
+        "FBP is not implemented yet. Use a conebeam geometry with the `bp` "
+        "function, or file an issue.")
+
+
+def fdk(projections: Any, projection_geometry: Any,
+        volume_geometry: VolumeGeometry, filter: Any = "ramlak",
+        **bp_kwargs):
+    """Feldkamp-Davis-Kress (``algo.py:169-223``): pre-weight, row filter,
+    adjoint-scaled voxel-driven backprojection (no extra FDK normalisation,
+    as in the reference)."""
+    for p in projection_geometry:
+        if p.beam != Beam.CONE:
+            raise NotImplementedError(
+                "Only cone beam geometry is supported at the moment.")
+    if "kernel" not in bp_kwargs:
+        bp_kwargs.update(kernel=VoxelDrivenConeBP())
+    return bp(projections, projection_geometry, volume_geometry,
+              filter=filter, **bp_kwargs)
+
+
+class _SirtState:
+    """Device buffers and launch arguments of one SIRT problem."""
+
+    def __init__(self, projections, projection_geometry, volume_geometry,
+                 mask, min_constraint, max_constraint):
+        import torch
+        self.pg = list(projection_geometry)
+        self.vg = volume_geometry
+        n = len(self.pg)
+        rows = int(self.pg[0].detector.rows)
+        cols = int(self.pg[0].detector.cols)
+        self.n, self.rows, self.cols = n, rows, cols
+        dev = default_device()
+        if isinstance(projections, (list, tuple)):
+            y = torch.stack([as_device_array(p) for p in projections])
+        else:
+            y = as_device_array(projections)
+        if tuple(y.shape) != (n, rows, cols):
+            raise ValueError(f"projections of shape {tuple(y.shape)} do not "
+                             f"match the geometry ({n}, {rows}, {cols})")
+        self.y = y
+        shape = tuple(int(s) for s in volume_geometry.shape)
+        self.shape = shape
+        self.fptor = ForwardProjector()
+        self.fptor.projection_geometry = self.pg
+        self.fptor.volume_geometry = volume_geometry
+        self.bptor = BackProjector(texture_type="pitch2d")
+        self.bptor.projection_geometry = self.pg
+        self.bptor.volume_geometry = volume_geometry
+        self.x = torch.zeros(shape, dtype=torch.float32, device=dev)
+        # C = 1/BP(1), R = 1/FP(1) with inf -> 0 (algo.py:320-333)
+        C = torch.zeros(shape, dtype=torch.float32, device=dev)
+        self.bptor.projections = torch.ones((n, rows, cols),
+                                            dtype=torch.float32, device=dev)
+        self.bptor.volume = C
+        self.bptor()
+        R = torch.empty((n, rows, cols), dtype=torch.float32, device=dev)
+        self.fptor.volume = torch.ones(shape, dtype=torch.float32,
+                                       device=dev)
+        self.fptor.projections = R
+        self.fptor()
+        lib = _native.lib()
+        st = _native.stream_ptr()
+        _native.check(lib.kk_recip_or_zero(_native._vp(C.data_ptr()),
+                                           C.numel(), st))
+        _native.check(lib.kk_recip_or_zero(_native._vp(R.data_ptr()),
+                                           R.numel(), st))
+        self.C, self.R = C, R
+        # textures: x for the FP, residual for the BP (halo stays zero)
+        self.fptor.volume = self.x
+        self.xtex = self.fptor._texture
+        self.rtex = self.bptor._textures
+        _native.check(lib.kk_fill(_native._vp(self.rtex.ptr),
+                                  self.rtex.data.numel(), 0.0, st))
+        self.res = self.rtex.interior()
+        self.mask = None if mask is None else as_device_array(mask)
+        self.lo = 0.0 if min_constraint is None else float(min_constraint)
+        self.hi = 0.0 if max_constraint is None else float(max_constraint)
+        self.clip_lo = 0 if min_constraint is None else 1
+        self.clip_hi = 0 if max_constraint is None else 1
+        self.scale = float(volume_geometry.voxel_volume())
+        xi = self.xtex.interior()
+        self.xtex_origin = xi.data_ptr()
+        self.xtex_strides = _world_strides(xi, (0, 1, 2))
+
+    def iterate(self):
+        lib = _native.lib()
+        st = _native.stream_ptr()
+        fk, bk = self.fptor.kernel, self.bptor.kernel
+        _native.check(lib.kk_sirt_fp_residual(
+            fk._plan.ptr, _native._vp(self.xtex.ptr),
+            _native._vp(self.y.data_ptr()), _native.i64(self.y.stride()),
+            _native._vp(self.R.data_ptr()), _native.i64(self.R.stride()),
+            _native._vp(self.res.data_ptr()), _native.i64(self.res.stride()),
+            self.rows, self.cols, st))
+        mask_ptr = None if self.mask is None else _native._vp(
+            self.mask.data_ptr())
+        mask_st = None if self.mask is None else _native.i64(
+            self.mask.stride())
+        _native.check(lib.kk_sirt_bp_update(
+            bk._plan.ptr, _native._vp(self.rtex.ptr), _native.i32((0, 1, 2)),
+            self.n, self.rows, self.cols, _native._vp(self.x.data_ptr()),
+            _native.i32(self.shape), _native.i64(self.x.stride()),
+            _native._vp(self.C.data_ptr()), _native.i64(self.C.stride()),
+            mask_ptr, mask_st, self.lo, self.hi, self.clip_lo, self.clip_hi,
+            _native._vp(self.xtex_origin), _native.i64(self.xtex_strides),
+            self.scale, st))
+
+
+def sirt(projections: Any, projection_geometry: Any,
+         volume_geometry: VolumeGeometry, iters: int = 100,
+         verbose: bool = True, mask: Any = None,
+         min_constraint: float = None, max_constraint: float = None,
+         return_gpu: bool = False, callback: callable = None,
+         residual_every: int = 10):
+    """Simultaneous Iterative Reconstruction Technique (``algo.py:226-368``).
+
+    ``x <- clip(mask * (x - C * A^T (R * (A x - y))))`` from ``x = 0`` with
+    ``C = 1/A^T 1`` and ``R = 1/A 1``.  ``callback(i, x, residual)`` gets
+    device tensors.  The residual norm is only computed (a device-to-host
+    sync) every ``residual_every`` iterations when ``verbose``.
+    """
+    if len(projections) != len(projection_geometry):
+        raise ValueError("Number of projections does not match number of "
+                         "geometries.")
+    state = _SirtState(projections, projection_geometry, volume_geometry,
+                       mask, min_constraint, max_constraint)
+    bar = range(iters)
+    if verbose:
+        from tqdm import tqdm
+        bar = tqdm(bar, desc="SIRT")
+    res_norm = ""
+    for i in bar:
+        state.iterate()
+        if verbose and residual_every is not None:
+            if i % residual_every == 0:
+                res_norm = str(float(state.res.float().norm()))
+            bar.set_description(f"SIRT: {res_norm}")
+        if callback is not None:
+            callback(i, state.x, state.res)
+    if return_gpu:
+        return state.x
+    return _to_host(state.x)

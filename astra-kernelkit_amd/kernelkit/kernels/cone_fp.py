@@ -1,0 +1,275 @@
+"""Ray-driven cone-beam forward projection kernel (Joseph's method).
+
+Drop-in for ``kernelkit/kernels/cone_fp.py`` of the reference
+(``RayDrivenConeFP``, :18-363): same constructor, ``compile``,
+``geoms2params``, ``set_params`` and ``__call__`` contract, backed by the
+gfx950 kernel ``kk_cone_fp`` (one launch for all angles and slices).
+"""
+from __future__ import annotations
+
+import copy
+
+import numpy as np
+
+from kernelkit import _native
+from kernelkit.geom import normalize_
+from kernelkit.geom.proj import ProjectionGeometrySequence
+from kernelkit.geom.vol import VolumeGeometry
+from kernelkit.kernel import (BaseKernel, KernelMemoryUnpreparedException,
+                              KernelNotCompiledException, Texture,
+                              is_device_array)
+
+
+class FPParams(tuple):
+    """The 12 per-angle arrays of :meth:`RayDrivenConeFP.geoms2params`
+    (float64), plus ``ray_axes``: the per-angle major axis taken from the
+    normalised detector centre."""
+
+    ray_axes: np.ndarray | None = None
+
+
+def _logical_strides(tensor, axes) -> list:
+    """(angle, row, col) element strides of an array whose axis k holds
+    logical axis ``axes[k]``."""
+    st = [0, 0, 0]
+    for k, ax in enumerate(axes):
+        st[ax] = tensor.stride(k)
+    return st
+
+
+class RayDrivenConeFP(BaseKernel):
+    """Ray-driven cone-beam forward projection kernel."""
+
+    FLOAT_DTYPE = np.float32
+    SUPPORTED_DTYPES = [np.float32]
+
+    pixels_per_block = (1, 32, 32)
+    slices_per_block = 16
+
+    def __init__(self, pixels_per_block: tuple = None,
+                 slices_per_block: int = None,
+                 volume_axes: tuple = (0, 1, 2),
+                 projection_axes: tuple = (0, 1, 2)):
+        """Cone-beam forward projection kernel.
+
+        ``volume_axes[k]`` is the world axis (x=0, y=1, z=2) of array axis
+        ``k`` of the volume; ``projection_axes[k]`` is the logical axis
+        (angle=0, row=1, col=2) of array axis ``k`` of the projections.
+        ``pixels_per_block`` and ``slices_per_block`` are accepted for API
+        compatibility; the gfx950 kernel processes all slices of a ray in
+        one pass and picks its own tile shape.
+        """
+        super().__init__()
+        if pixels_per_block is not None:
+            self.pixels_per_block = tuple(pixels_per_block)
+        if slices_per_block is not None:
+            self.slices_per_block = int(slices_per_block)
+        for name, ax in (("volume_axes", volume_axes),
+                         ("projection_axes", projection_axes)):
+            if sorted(ax) != [0, 1, 2]:
+                raise ValueError(f"`{name}` must be a permutation of "
+                                 f"(0, 1, 2), got {ax}.")
+        self._vol_axs = tuple(volume_axes)
+        self._proj_axs = tuple(projection_axes)
+        self._params_are_set = False
+        self._params = None
+        self._uploaded_key = None
+
+    @property
+    def volume_axes(self) -> tuple:
+        return self._vol_axs
+
+    @property
+    def projection_axes(self) -> tuple:
+        return self._proj_axs
+
+    def compile(self, max_projs: int | None = None):
+        """Load the gfx950 library and create this kernel's plan.
+
+        ``max_projs`` (optional) caps the number of angles like the
+        reference's constant-memory size (``kernels/cone_fp.py:88-100``);
+        by default there is no cap (tables live in device memory).
+        """
+        return self._compile({"max_projs": max_projs,
+                              "volume_axes": self._vol_axs,
+                              "projection_axes": self._proj_axs})
+
+    @staticmethod
+    def _geom_axes(projection_geometry: ProjectionGeometrySequence):
+        """Per-angle ray axis ``argmax |source - detector|``
+        (``kernels/cone_fp.py:264-285``)."""
+        return np.argmax(np.abs(projection_geometry.source_position
+                                - projection_geometry.detector_position),
+                         axis=1).astype(np.int32)
+
+    @staticmethod
+    def _output_scale(voxel_size) -> list:
+        """``[scale1, scale2, outScale]`` per ray axis
+        (``kernels/cone_fp.py:287-300``, including its anisotropic table)."""
+        v = [float(x) for x in voxel_size]
+        if abs(v[0] / v[1] - 1) > 1e-4 or abs(v[0] / v[2] - 1) > 1e-4:
+            return [[(v[1] / v[0]) ** 2, (v[2] / v[0]) ** 2, v[0] ** 2],
+                    [(v[0] / v[1]) ** 2, (v[2] / v[1]) ** 2, v[0] * v[1]],
+                    [(v[0] / v[2]) ** 2, (v[1] / v[2]) ** 2, v[0] * v[2]]]
+        return [[1.0, 1.0, v[0]]] * 3
+
+    @staticmethod
+    def geoms2params(projection_geometry, volume_geometry: VolumeGeometry):
+        """Per-angle kernel parameters in the volume's voxel frame.
+
+        Returns 12 float64 arrays ``(src.x, src.y, src.z, corner.x, corner.y,
+        corner.z, U.x, U.y, U.z, V.x, V.y, V.z)`` as in
+        ``kernels/cone_fp.py:302-339``, where ``U``/``V`` are the column/row
+        pixel steps.  Unlike the reference the input geometry is not
+        normalised in place, and everything stays float64 (the library
+        rounds to float32 only after shifting to the pixel-centre/source
+        frame).
+        """
+        if not isinstance(projection_geometry, ProjectionGeometrySequence):
+            projection_geometry = ProjectionGeometrySequence.fromList(
+                projection_geometry)
+        seq = copy.deepcopy(projection_geometry)
+        normalize_(seq, volume_geometry)
+        src = seq.source_position
+        corner = seq.detector_extent_min
+        U = seq.u * seq.detector.pixel_width[:, None]
+        V = seq.v * seq.detector.pixel_height[:, None]
+        out = FPParams(tuple(np.ascontiguousarray(a[:, k])
+                             for a in (src, corner, U, V) for k in range(3)))
+        out.ray_axes = RayDrivenConeFP._geom_axes(seq)
+        return out
+
+    def set_params(self, params, projection_geometry=None,
+                   volume_geometry=None):
+        """Store the per-angle parameters (``kernels/cone_fp.py:341-363``).
+
+        The device table also depends on the volume shape and detector size;
+        it is uploaded here when ``projection_geometry`` and
+        ``volume_geometry`` are given, else at the first call.
+        """
+        if not self.is_compiled:
+            raise KernelNotCompiledException(
+                f"Compile {self.__class__.__name__} before setting its "
+                "parameters.")
+        if len(params) != 12:
+            raise ValueError("Expected 12 parameter arrays.")
+        arr = np.ascontiguousarray(
+            np.stack([np.asarray(p, dtype=np.float64).ravel()
+                      for p in params]))
+        n = arr.shape[1]
+        max_projs = self._compiled_template_kwargs.get("max_projs")
+        if max_projs is not None and n > max_projs:
+            raise ValueError(
+                f"Number of projections, {n}, exceeds the maximum of the "
+                f"compiled kernel, namely {max_projs}. Please recompile the "
+                "kernel with a higher `max_projs`.")
+        axes = getattr(params, "ray_axes", None)
+        self._params = arr
+        self._ray_axes = (None if axes is None
+                          else np.ascontiguousarray(axes, dtype=np.int32))
+        self._params_are_set = True
+        self._uploaded_key = None
+        if projection_geometry is not None and volume_geometry is not None:
+            det = projection_geometry[0].detector
+            self._upload(int(det.rows), int(det.cols), volume_geometry)
+
+    def _upload(self, rows: int, cols: int, vg: VolumeGeometry) -> None:
+        key = (rows, cols, tuple(vg.shape), tuple(vg.voxel_size))
+        if key == self._uploaded_key:
+            return
+        n = self._params.shape[1]
+        axes_ptr = (None if self._ray_axes is None
+                    else _native.i32_ptr(self._ray_axes))
+        _native.check(_native.lib().kk_set_params_fp(
+            self._plan.ptr, _native.f64_ptr(self._params), n, axes_ptr,
+            rows, cols, _native.i32(vg.shape), _native.f64(vg.voxel_size),
+            _native.i32(self._vol_axs)))
+        self._uploaded_key = key
+
+    def __call__(self, volume_texture: Texture, projection_geometry,
+                 volume_geometry: VolumeGeometry, projections,
+                 projections_pitch: tuple | None = None, *,
+                 additive: bool = True, stream=None):
+        """Forward-project the volume texture into ``projections``.
+
+        ``projections`` is a device tensor in ``projection_axes`` order (row
+        pitch allowed through its strides) or a list of 2-D tensors, one per
+        index of the first axis.  Like the reference kernel the result is
+        accumulated (``+=``, ``cuda/cone_fp.cu:201-204``) unless
+        ``additive=False``.  ``projections_pitch`` is accepted for API
+        compatibility; strides carry the pitch.
+        """
+        if volume_texture is None:
+            raise ValueError("`volume_texture` must be given.")
+        if not self.is_compiled:
+            raise KernelNotCompiledException(
+                "Please compile the kernel with `compile()`.")
+        if not self._params_are_set:
+            raise KernelMemoryUnpreparedException(
+                "Please set the projection geometry with `set_params()`.")
+        if not isinstance(volume_texture, Texture):
+            raise TypeError("`volume_texture` must be a kernelkit Texture.")
+        n = self._params.shape[1]
+        if len(projection_geometry) != n:
+            raise ValueError(
+                f"{len(projection_geometry)} projection geometries but "
+                f"parameters for {n} angles.")
+        det = projection_geometry[0].detector
+        rows, cols = int(det.rows), int(det.cols)
+        expect_vol = tuple(int(volume_geometry.shape[i])
+                           for i in self._vol_axs)
+        if tuple(volume_texture.shape) != expect_vol:
+            raise ValueError(
+                f"Volume texture shape {volume_texture.shape} does not match "
+                f"volume geometry {tuple(volume_geometry.shape)} with axes "
+                f"{self._vol_axs}.")
+        self._upload(rows, cols, volume_geometry)
+        logical = (n, rows, cols)
+        expect = tuple(logical[i] for i in self._proj_axs)
+        if isinstance(projections, (list, tuple)):
+            import torch
+            if len(projections) != expect[0]:
+                raise ValueError(
+                    "The length of projection array must match the number of "
+                    f"entries along the first axis of `projection_axes` "
+                    f"{self._proj_axs}, {expect[0]}, but got "
+                    f"{len(projections)}.")
+            for p in projections:
+                if not is_device_array(p):
+                    raise TypeError("Projection data must be device tensors, "
+                                    f"but got {type(p)}.")
+                if p.dtype != torch.float32:
+                    raise NotImplementedError(
+                        "Currently there is only support for dtype float32, "
+                        f"but got {p.dtype}.")
+            tmp = torch.empty(expect, dtype=torch.float32,
+                              device=projections[0].device)
+            if additive:
+                for k, p in enumerate(projections):
+                    tmp[k].copy_(p)
+            self._launch(volume_texture, tmp, rows, cols, additive, stream)
+            for k, p in enumerate(projections):
+                p.copy_(tmp[k])
+            return
+        if not is_device_array(projections):
+            raise TypeError("Projection data must be a device tensor or a "
+                            f"list of them, got {type(projections)}.")
+        import torch
+        if projections.dtype != torch.float32:
+            raise NotImplementedError(
+                "Currently there is only support for dtype float32, but got "
+                f"{projections.dtype}.")
+        if tuple(projections.shape) != expect:
+            raise ValueError(
+                f"Projections shape {tuple(projections.shape)} does not match "
+                f"{expect} (angles, rows, cols permuted by "
+                f"{self._proj_axs}).")
+        self._launch(volume_texture, projections, rows, cols, additive,
+                     stream)
+
+    def _launch(self, tex: Texture, proj, rows, cols, additive, stream):
+        strides = _logical_strides(proj, self._proj_axs)
+        _native.check(_native.lib().kk_cone_fp(
+            self._plan.ptr, _native._vp(tex.ptr), _native._vp(proj.data_ptr()),
+            _native.i64(strides), rows, cols, 1 if additive else 0,
+            _native.stream_ptr(stream)))

@@ -1,0 +1,139 @@
+"""Linear operators built on a forward projector and a backprojector.
+
+Drop-in for ``kernelkit/operator.py`` of the reference: ``BaseOperator``
+(:10-33), ``ProjectorOperator`` with its ``.T`` adjoint wrapper (:36-146) and
+``XrayTransform`` (:149-233).  Inputs/outputs are device tensors.
+"""
+from __future__ import annotations
+
+from abc import ABC, abstractmethod
+from typing import Any
+
+from kernelkit.projector import (BackProjector, BaseProjector,  # noqa: F401
+                                 ForwardProjector)
+
+
+def _zeros(shape):
+    import torch
+    from kernelkit.kernel import default_device
+    return torch.zeros(shape, dtype=torch.float32, device=default_device())
+
+
+class BaseOperator(ABC):
+    """A linear operator with domain/range shapes and an adjoint."""
+
+    @property
+    @abstractmethod
+    def domain_shape(self) -> tuple: ...
+
+    @property
+    @abstractmethod
+    def range_shape(self) -> tuple: ...
+
+    @abstractmethod
+    def __call__(self, input, out=None): ...
+
+    @property
+    def T(self):
+        raise NotImplementedError
+
+
+class ProjectorOperator(BaseOperator, ABC):
+    """``A : X -> Y`` from a forward projector; ``A.T`` uses the
+    backprojector (``operator.py:36-146``)."""
+
+    def __init__(self, projector: BaseProjector,
+                 backprojector: BaseProjector):
+        self.projector = projector
+        self.backprojector = backprojector
+
+    @property
+    def domain_shape(self) -> tuple:
+        vg = self.projector.volume_geometry
+        return tuple(int(vg.shape[i]) for i in self.projector.volume_axes)
+
+    @property
+    def range_shape(self) -> tuple:
+        pg = self.projector.projection_geometry
+        det = pg[0].detector
+        full = (len(pg), int(det.rows), int(det.cols))
+        return tuple(full[i] for i in self.projector.projection_axes)
+
+    def __call__(self, input, out=None, additive=False):
+        """``out (+)= A(input)``; allocates a zero output if none is
+        given."""
+        if tuple(input.shape) != self.domain_shape:
+            raise ValueError(
+                f"Input shape {tuple(input.shape)} does not match domain "
+                f"shape {self.domain_shape} of operator.")
+        if out is None:
+            out = _zeros(self.range_shape)
+        self.projector.volume = input
+        self.projector.projections = out
+        self.projector(additive=additive)
+        return out
+
+    @property
+    def T(self):
+        outer = self
+
+        class _Adjoint(BaseOperator):
+            """``A^* : Y -> X`` (the unmatched voxel-driven BP)."""
+
+            def __init__(self):
+                self.projector = outer.backprojector
+                self.backprojector = outer.projector
+
+            @property
+            def domain_shape(self) -> tuple:
+                return outer.range_shape
+
+            @property
+            def range_shape(self) -> tuple:
+                return outer.domain_shape
+
+            def __call__(self, input, out=None, additive=False):
+                if tuple(input.shape) != self.domain_shape:
+                    raise ValueError(
+                        f"Input shape {tuple(input.shape)} does not match "
+                        f"domain shape {self.domain_shape} of operator.")
+                if out is None:
+                    out = _zeros(self.range_shape)
+                self.projector.projections = input
+                self.projector.volume = out
+                self.projector(additive)
+                return out
+
+            @property
+            def T(self):
+                return outer
+
+        if not hasattr(self, "_T"):
+            self._T = _Adjoint()
+        return self._T
+
+
+class XrayTransform(ProjectorOperator):
+    """The cone-beam X-ray transform (``operator.py:149-233``)."""
+
+    def __init__(self, projection_geometry: Any, volume_geometry: Any,
+                 volume_axes: tuple = (0, 1, 2),
+                 projection_axes: tuple = (0, 1, 2),
+                 use_toolbox: bool = False, fp_kwargs=None, bp_kwargs=None):
+        if use_toolbox:
+            raise NotImplementedError(
+                "ASTRA Toolbox projectors are not available in the MI355X "
+                "build (`use_toolbox=True`).")
+        fp_kwargs = fp_kwargs or {}
+        bp_kwargs = bp_kwargs or {}
+        projector = ForwardProjector(volume_axes=volume_axes,
+                                     projection_axes=projection_axes,
+                                     **fp_kwargs)
+        backprojector = BackProjector(volume_axes=volume_axes,
+                                      projection_axes=projection_axes,
+                                      **bp_kwargs)
+        projector.volume_geometry = volume_geometry
+        projector.projection_geometry = projection_geometry
+        backprojector.volume_geometry = volume_geometry
+        backprojector.projection_geometry = projection_geometry
+        super().__init__(projector, backprojector)

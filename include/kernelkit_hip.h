@@ -1,0 +1,191 @@
+/*
+ * kernelkit_hip.h -- C ABI of the MI355X (gfx950) cone-beam operators.
+ *
+ * Plain C: pointers, sizes and status codes; no torch or C++ types.  Every
+ * device-side call is asynchronous on the given HIP stream (a hipStream_t
+ * passed as void*; NULL = the null stream), allocates nothing and never
+ * synchronises, so it can be captured in a HIP graph.  Only the
+ * kk_set_params_* calls allocate (a per-plan device table) and synchronise.
+ *
+ * Errors: every int-returning function returns KK_OK (0) or a negative
+ * KK_ERR_* code, and records a message for kk_last_error() (thread-local).
+ * Validation happens before any launch.
+ *
+ * Conventions (mirroring the reference, paths relative to its repository):
+ *   world axes (x, y, z); volumes are addressed by world strides;
+ *   projections by (angle, row, col) strides, in elements (floats);
+ *   kernel parameters are the reference's kernel parameters, in float64,
+ *   in the volume's voxel frame (kernelkit/geom/__init__.py:305-330).
+ */
+#ifndef KERNELKIT_HIP_H
+#define KERNELKIT_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KK_OK 0
+#define KK_ERR_INVALID (-1)     /* bad argument (ValueError)                */
+#define KK_ERR_HIP (-2)         /* HIP runtime failure                       */
+#define KK_ERR_NOT_SET (-3)     /* parameters not set (KernelMemoryUnprepared) */
+#define KK_ERR_SHAPE (-4)       /* shape mismatch (ValueError)               */
+#define KK_ERR_UNSUPPORTED (-5) /* unsupported case (NotImplementedError)    */
+
+/* Padded "texture" layouts: every axis gets KK_HALO_LO zero elements before
+ * and KK_HALO_HI after the data (replaces the CUDA texture's border mode,
+ * kernelkit/kernel.py:22-32). */
+#define KK_HALO_LO 1
+#define KK_HALO_HI 2
+
+typedef struct kk_plan_s kk_plan; /* opaque per-geometry device tables */
+
+/* Last error message of the calling thread ("" if none). */
+const char* kk_last_error(void);
+/* Library version string. */
+const char* kk_version(void);
+
+/* Create/destroy a plan (replaces the NVRTC module holding __constant__
+ * geometry, kernelkit/kernel.py:122-147, :188-226).  The plan binds to the
+ * current HIP device. */
+int kk_plan_create(kk_plan** out);
+int kk_plan_destroy(kk_plan* plan);
+
+/* RayDrivenConeFP.set_params (kernelkit/kernels/cone_fp.py:341-363).
+ * params: 12 x n_angles float64, structure of arrays, in the order returned
+ * by RayDrivenConeFP.geoms2params (kernels/cone_fp.py:336-339):
+ * src.xyz, det_corner.xyz, U.xyz (column step), V.xyz (row step), in voxel
+ * units.  ray_axes (nullable): per-angle ray axis 0/1/2, the reference's
+ * argmax |source - detector centre| (kernels/cone_fp.py:264-285); when NULL
+ * it is computed here from the corner plus rows/cols.  vol_shape and
+ * voxel_size are world (x, y, z); volume_axes[k] is the world axis of array
+ * axis k of the volume (kernels/cone_fp.py:43-47).  Together they fix the
+ * texture strides and the output scale (kernels/cone_fp.py:287-300). */
+int kk_set_params_fp(kk_plan* plan, const double* params, int32_t n_angles,
+                     const int32_t* ray_axes, int32_t rows, int32_t cols,
+                     const int32_t vol_shape[3], const double voxel_size[3],
+                     const int32_t volume_axes[3]);
+
+/* VoxelDrivenConeBP.set_params (kernelkit/kernels/cone_bp.py:168-190).
+ * params: n_angles x 12 float64 as returned by VoxelDrivenConeBP.geoms2params
+ * (kernels/cone_bp.py:401-421): per angle numU.xyzw, numV.xyzw, den.xyzw. */
+int kk_set_params_bp(kk_plan* plan, const double* params, int32_t n_angles);
+
+/* Number of floats of the padded texture of a 3-D array of `shape`. */
+int64_t kk_texture_elems(const int64_t shape[3]);
+
+/* Array -> padded texture (the CUDA-array copy of copy_to_texture,
+ * kernelkit/kernel.py:47-119).  The texture keeps the array's own axis
+ * order, C-contiguous, with a zero halo of KK_HALO_LO/KK_HALO_HI on every
+ * axis; `strides` are the source's element strides (views, pitched rows and
+ * permutations are fine).  The whole texture is written, halo included. */
+int kk_array_to_texture(const float* src, const int64_t shape[3],
+                        const int64_t strides[3], float* tex, void* stream);
+
+/* value-fill of n floats (a kernel, not hipMemsetAsync: graph-safe). */
+int kk_fill(float* ptr, int64_t n, float value, void* stream);
+
+/* Ray-driven Joseph forward projection, all angles in one launch
+ * (cone_fp<DIR_X|Y|Z>, kernelkit/cuda/cone_fp.cu:73-207, launched by
+ * kernels/cone_fp.py:195-262).  vol_tex: texture of the volume array whose
+ * shape/axes were given to kk_set_params_fp.  proj: output addressed by
+ * (angle, row, col) element strides (any projection_axes or pitch).
+ * additive != 0: proj += result; else proj = result (the reference
+ * zero-fills first, projector.py:337-343). */
+int kk_cone_fp(const kk_plan* plan, const float* vol_tex, float* proj,
+               const int64_t proj_strides[3], int32_t rows, int32_t cols,
+               int32_t additive, void* stream);
+
+/* Voxel-driven backprojection, all angles in one launch (cone_bp,
+ * kernelkit/cuda/cone_bp.cu:35-159, kernels/cone_bp.py:292-302).
+ * proj_tex: texture of the projection array, whose array axis k is
+ * (angle, row, col)[projection_axes[k]] (cone_bp.cu:112-138); n_angles must
+ * equal the parameter count.  vol: output addressed by world (x, y, z)
+ * element strides; output_scale: the voxel volume (kernels/cone_bp.py:294).
+ * additive != 0: vol += result; else vol = result (projector.py:586-587). */
+int kk_cone_bp(const kk_plan* plan, const float* proj_tex,
+               const int32_t projection_axes[3], int32_t n_angles,
+               int32_t rows, int32_t cols, float* vol,
+               const int32_t vol_shape[3], const int64_t vol_strides[3],
+               float output_scale, int32_t additive, void* stream);
+
+/* ---- host-only introspection (no device calls) ---------------------- */
+
+/* The per-angle device records that kk_set_params_fp / kk_set_params_bp
+ * would upload, written to `out` (n_angles x 128 bytes for the FP table,
+ * n_angles x 64 bytes for the BP table; layouts FpAngle / BpAngle of
+ * csrc/kk_internal.h).  Used by CPU tests to check the table arithmetic. */
+int kk_fp_table(const double* params, int32_t n_angles,
+                const int32_t* ray_axes, int32_t rows, int32_t cols,
+                const int32_t vol_shape[3], const double voxel_size[3],
+                const int32_t volume_axes[3], void* out);
+int kk_bp_table(const double* params, int32_t n_angles, void* out);
+
+/* ---- SIRT (kernelkit/algo.py:226-368), fused ------------------------- */
+
+/* out = (FP(x) - y) * R for every ray: the residual step of one SIRT
+ * iteration (algo.py:339-344) fused into the forward projection epilogue.
+ * y, R, out: (angle, row, col) element strides; out may be the interior of
+ * a projection texture. */
+int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
+                        const float* y, const int64_t y_strides[3],
+                        const float* R, const int64_t R_strides[3],
+                        float* out, const int64_t out_strides[3],
+                        int32_t rows, int32_t cols, void* stream);
+
+/* x = clip(mask * (x - C * BP(res))) (algo.py:356-362), fused into the
+ * backprojection epilogue.  mask (nullable), C, x: world strides; clip_lo /
+ * clip_hi select the min/max constraints.  x_tex (nullable): the FP texture
+ * of x at voxel (0, 0, 0) with world strides x_tex_strides, refreshed with
+ * the new x so the next forward projection needs no copy. */
+int kk_sirt_bp_update(const kk_plan* plan, const float* res_tex,
+                      const int32_t projection_axes[3], int32_t n_angles,
+                      int32_t rows, int32_t cols, float* x,
+                      const int32_t vol_shape[3], const int64_t x_strides[3],
+                      const float* C, const int64_t C_strides[3],
+                      const float* mask, const int64_t mask_strides[3],
+                      float lo, float hi, int32_t clip_lo, int32_t clip_hi,
+                      float* x_tex, const int64_t x_tex_strides[3],
+                      float output_scale, void* stream);
+
+/* p = 1/p with +inf mapped to 0, n contiguous floats (algo.py:324-332). */
+int kk_recip_or_zero(float* ptr, int64_t n, void* stream);
+
+/* ---- FDK processing (kernelkit/processing.py) ------------------------- */
+
+typedef struct kk_filter_s kk_filter; /* opaque filter tables */
+
+int kk_filter_create(kk_filter** out);
+int kk_filter_destroy(kk_filter* filter);
+
+/* Frequency response H (float64, length n, a power of two <= 8192), the
+ * output of processing._filter (processing.py:44-93). */
+int kk_filter_set_response(kk_filter* filter, const double* H, int32_t n);
+
+/* Per-angle pre-weight geometry, n_angles x 10 float64 in world units:
+ * (d_min - s).xyz, (u * pixel_width).xyz, (v * pixel_height).xyz, |d - s|
+ * (processing.py:170-188). */
+int kk_filter_set_preweight(kk_filter* filter, const double* params,
+                            int32_t n_angles);
+
+/* In-place FDK cosine pre-weight (processing.preweight, :149-188). */
+int kk_preweight(const kk_filter* filter, float* proj, int32_t n_angles,
+                 int32_t rows, int32_t cols, const int64_t strides[3],
+                 void* stream);
+
+/* Row filter (processing.filter, :96-126): zero-pad each detector row to
+ * the response length, multiply by H in frequency space, keep the first
+ * cols values.  in/out: (angle, row, col) strides, may alias exactly (in
+ * place).  fuse_preweight != 0 applies kk_preweight's factor on load. */
+int kk_filter_rows(const kk_filter* filter, const float* in,
+                   const int64_t in_strides[3], float* out,
+                   const int64_t out_strides[3], int32_t n_angles,
+                   int32_t rows, int32_t cols, int32_t fuse_preweight,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KERNELKIT_HIP_H */
