@@ -28,6 +28,14 @@ class FPParams(tuple):
     ray_axes: np.ndarray | None = None
 
 
+def detector_shape(projection_geometry) -> tuple:
+    """(rows, cols) of the first angle's detector (the reference uses angle
+    0 for every angle, ``kernels/cone_fp.py:199-200``)."""
+    det = projection_geometry[0].detector
+    return (int(np.asarray(det.rows).ravel()[0]),
+            int(np.asarray(det.cols).ravel()[0]))
+
+
 def _logical_strides(tensor, axes) -> list:
     """(angle, row, col) element strides of an array whose axis k holds
     logical axis ``axes[k]``."""
@@ -170,8 +178,8 @@ class RayDrivenConeFP(BaseKernel):
         self._params_are_set = True
         self._uploaded_key = None
         if projection_geometry is not None and volume_geometry is not None:
-            det = projection_geometry[0].detector
-            self._upload(int(det.rows), int(det.cols), volume_geometry)
+            rows, cols = detector_shape(projection_geometry)
+            self._upload(rows, cols, volume_geometry)
 
     def _upload(self, rows: int, cols: int, vg: VolumeGeometry) -> None:
         key = (rows, cols, tuple(vg.shape), tuple(vg.voxel_size))
@@ -214,8 +222,7 @@ class RayDrivenConeFP(BaseKernel):
             raise ValueError(
                 f"{len(projection_geometry)} projection geometries but "
                 f"parameters for {n} angles.")
-        det = projection_geometry[0].detector
-        rows, cols = int(det.rows), int(det.cols)
+        rows, cols = detector_shape(projection_geometry)
         expect_vol = tuple(int(volume_geometry.shape[i])
                            for i in self._vol_axs)
         if tuple(volume_texture.shape) != expect_vol:

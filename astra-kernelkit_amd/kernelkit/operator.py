@@ -9,6 +9,7 @@ from __future__ import annotations
 from abc import ABC, abstractmethod
 from typing import Any
 
+from kernelkit.kernels.cone_fp import detector_shape
 from kernelkit.projector import (BackProjector, BaseProjector,  # noqa: F401
                                  ForwardProjector)
 
@@ -55,8 +56,7 @@ class ProjectorOperator(BaseOperator, ABC):
     @property
     def range_shape(self) -> tuple:
         pg = self.projector.projection_geometry
-        det = pg[0].detector
-        full = (len(pg), int(det.rows), int(det.cols))
+        full = (len(pg), *detector_shape(pg))
         return tuple(full[i] for i in self.projector.projection_axes)
 
     def __call__(self, input, out=None, additive=False):

@@ -274,6 +274,10 @@ class BackProjector(Projector):
             raise ValueError(
                 "`TEXTURE_TYPE` not supported. Please choose one of the "
                 f"following: {self.TEXTURE_TYPES}.")
+        if self._texture_type == "layered" and K.projection_axes[0] != 0:
+            # A CUDA layered texture needs the angle axis first; the padded
+            # texture here does not, so serve it as a 3-D texture.
+            self._texture_type = "array"
         type2texture = {
             "pitch2d": VoxelDrivenConeBP.InterpolationMethod.Tex2D,
             "array2d": VoxelDrivenConeBP.InterpolationMethod.Tex2D,
