@@ -49,15 +49,17 @@ struct BpArgs {
   int64_t xts0, xts1, xts2;  // its world strides
 };
 
-// Two horizontally adjacent detector taps (columns u0, u0+1).
+// Two horizontally adjacent detector taps (columns u0, u0+1) at a
+// wave-uniform base plus a per-lane byte offset.
 template <bool PAIR>
-__device__ __forceinline__ kk_float2u col_pair(const float* p, int32_t tcs) {
+__device__ __forceinline__ kk_float2u col_pair(const char* base, uint32_t off,
+                                               uint32_t tcb) {
   if constexpr (PAIR) {
-    return *reinterpret_cast<const kk_float2u*>(p);
+    return kk_ld2(base, off);
   } else {
     kk_float2u t;
-    t.x = p[0];
-    t.y = p[tcs];
+    t.x = kk_ld1(base, off);
+    t.y = kk_ld1(base + tcb, off);
     return t;
   }
 }
@@ -79,19 +81,21 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
   const float X = static_cast<float>(ix) - 0.5f * A.nx + 0.5f;
   const float Y = static_cast<float>(iy) - 0.5f * A.ny + 0.5f;
   const float Z0 = static_cast<float>(iz0) - 0.5f * A.nz + 0.5f;
-  const float rmax = static_cast<float>(A.rows);
-  const float cmax = static_cast<float>(A.cols);
-  const int32_t trs = A.trs;
-  const int32_t tcs = PAIR ? 1 : A.tcs;
+  const float rmax = static_cast<float>(A.rows + 1);
+  const float cmax = static_cast<float>(A.cols + 1);
+  const uint32_t trb = 4u * static_cast<uint32_t>(A.trs);
+  const uint32_t tcb = PAIR ? 4u : 4u * static_cast<uint32_t>(A.tcs);
 
   float acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.f;
 
-  const float* img =
-      A.tex + kk::kHaloLo * (A.tas + int64_t(trs) + int64_t(tcs));
-  for (int32_t a = 0; a < A.n_angles; ++a, img += A.tas) {
+  // padded image of angle 0; the halo offset is folded into the forms
+  const char* img = reinterpret_cast<const char*>(A.tex + A.tas);
+  const size_t tasb = 4 * static_cast<size_t>(A.tas);
+  for (int32_t a = 0; a < A.n_angles; ++a, img += tasb) {
     const kk::BpAngle& P = tab[a];
+    const char* img_r = img + trb;  // next detector row, uniform
     const float den = fmaf(Y, P.dy, fmaf(X, P.dx, P.dw));
     const float nu = fmaf(Y, P.nuy, fmaf(X, P.nux, P.nuw));
     const float nv = fmaf(Y, P.nvy, fmaf(X, P.nvx, P.nvw));
@@ -104,24 +108,33 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
       // (den = 0) has no projection: skip it
       if (!(isfinite(un) && isfinite(v0) && isfinite(gv))) continue;
       const float w = r * r;
-      const float u = kk_clamp(un, -1.f, cmax);
-      const float uf = floorf(u);
-      const float fu = u - uf;
-      const float* colp = img + static_cast<int32_t>(uf) * tcs;
+      const float u = kk_clamp(un, 0.f, cmax);
+      const float fu = __builtin_amdgcn_fractf(u);
+      const uint32_t offu = __umul24(static_cast<uint32_t>(u), tcb);
+      uint32_t off[K];
+      float fv[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const float v = kk_clamp(fmaf(static_cast<float>(k), gv, v0), -1.f,
+        const float v = kk_clamp(fmaf(static_cast<float>(k), gv, v0), 0.f,
                                  rmax);
-        const float vf = floorf(v);
-        const float fv = v - vf;
-        const float* p = colp + static_cast<int32_t>(vf) * trs;
-        const kk_float2u t0 = col_pair<PAIR>(p, tcs);
-        const kk_float2u t1 = col_pair<PAIR>(p + trs, tcs);
-        const float h0 = fmaf(fu, t0.y - t0.x, t0.x);
-        const float h1 = fmaf(fu, t1.y - t1.x, t1.x);
-        acc[k] = fmaf(w, fmaf(fv, h1 - h0, h0), acc[k]);
+        fv[k] = __builtin_amdgcn_fractf(v);
+        off[k] = __umul24(static_cast<uint32_t>(v), trb) + offu;
+      }
+      kk_float2u t0[K], t1[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        t0[k] = col_pair<PAIR>(img, off[k], tcb);
+        t1[k] = col_pair<PAIR>(img_r, off[k], tcb);
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float h0 = fmaf(fu, t0[k].y - t0[k].x, t0[k].x);
+        const float h1 = fmaf(fu, t1[k].y - t1[k].x, t1[k].x);
+        acc[k] = fmaf(w, fmaf(fv[k], h1 - h0, h0), acc[k]);
       }
     } else {
+      uint32_t off[K];
+      float fu[K], fv[K], w[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const float Z = Z0 + static_cast<float>(k);
@@ -129,19 +142,25 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
         const float un = fmaf(Z, P.nuz, nu) * r;
         const float vn = fmaf(Z, P.nvz, nv) * r;
         const bool ok = isfinite(un) && isfinite(vn);
-        const float u = kk_clamp(ok ? un : -1.f, -1.f, cmax);
-        const float v = kk_clamp(ok ? vn : -1.f, -1.f, rmax);
-        const float uf = floorf(u);
-        const float vf = floorf(v);
-        const float fu = u - uf;
-        const float fv = v - vf;
-        const float* p = img + (static_cast<int32_t>(vf) * trs +
-                                static_cast<int32_t>(uf) * tcs);
-        const kk_float2u t0 = col_pair<PAIR>(p, tcs);
-        const kk_float2u t1 = col_pair<PAIR>(p + trs, tcs);
-        const float h0 = fmaf(fu, t0.y - t0.x, t0.x);
-        const float h1 = fmaf(fu, t1.y - t1.x, t1.x);
-        acc[k] = fmaf(ok ? r * r : 0.f, fmaf(fv, h1 - h0, h0), acc[k]);
+        const float u = kk_clamp(ok ? un : 0.f, 0.f, cmax);
+        const float v = kk_clamp(ok ? vn : 0.f, 0.f, rmax);
+        fu[k] = __builtin_amdgcn_fractf(u);
+        fv[k] = __builtin_amdgcn_fractf(v);
+        off[k] = __umul24(static_cast<uint32_t>(v), trb) +
+                 __umul24(static_cast<uint32_t>(u), tcb);
+        w[k] = ok ? r * r : 0.f;
+      }
+      kk_float2u t0[K], t1[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        t0[k] = col_pair<PAIR>(img, off[k], tcb);
+        t1[k] = col_pair<PAIR>(img_r, off[k], tcb);
+      }
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float h0 = fmaf(fu[k], t0[k].y - t0[k].x, t0[k].x);
+        const float h1 = fmaf(fu[k], t1[k].y - t1[k].x, t1[k].x);
+        acc[k] = fmaf(w[k], fmaf(fv[k], h1 - h0, h0), acc[k]);
       }
     }
   }
@@ -214,9 +233,15 @@ int launch_cone_bp(const kk_plan* plan, const float* tex,
   const int64_t astride[3] = {text[1] * text[2], text[2], 1};
   int64_t lstride[3];
   for (int k = 0; k < 3; ++k) lstride[pax[k]] = astride[k];
-  if (lstride[1] >= (int64_t(1) << 31) / (rows + kHalo) ||
-      lstride[2] >= (int64_t(1) << 31) / (cols + kHalo))
-    return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: detector too large");
+  // in-image byte offsets (row <= rows + 2, col <= cols + 2) must fit 32
+  // bits and the row/column byte strides 24 bits (v_mul_u32_u24)
+  if (4 * ((rows + 2) * lstride[1] + (cols + 2) * lstride[2] + 2) >=
+          (int64_t(1) << 32) ||
+      4 * lstride[1] >= (int64_t(1) << 24) ||
+      4 * lstride[2] >= (int64_t(1) << 24))
+    return fail(KK_ERR_UNSUPPORTED,
+                "kk_cone_bp: projection layout too large for 32-bit "
+                "in-image offsets");
   if (mode == 2 && (ep.C == nullptr || ep.cs == nullptr))
     return fail(KK_ERR_INVALID, "kk_cone_bp: SIRT update needs C");
   BpArgs A{};

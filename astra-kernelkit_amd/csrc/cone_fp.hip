@@ -11,23 +11,33 @@
 // * one launch covers all angles and all slices (the reference issues one
 //   launch per angle group and 16-slice chunk and accumulates in global
 //   memory); each output pixel is written exactly once;
-// * the volume is read from a zero-halo texture [x][y][z] (z fastest), so a
-//   sample is: two FMAs for its coordinates, a clamp into the halo, and four
-//   taps with no bounds tests; when the c axis is z the two c-neighbours are
-//   one 8-byte load;
+// * the volume is read from a zero-halo texture, so a sample is: one FMA per
+//   coordinate, a clamp into the halo, v_cvt_u32 + v_fract for the tap index
+//   and weight, and four taps with no bounds tests; when the c axis is the
+//   texture's contiguous axis the two c-neighbours are one 8-byte load;
 // * sample coordinates are evaluated directly, x_b(i) = a1*i + beta, instead
 //   of the reference's running sums, so the per-sample error does not grow
 //   along the ray;
-// * each ray loops only over the slices where it can touch the volume;
-// * lanes of a wavefront are consecutive detector rows of one column: for
-//   rotation about z they sample consecutive z voxels of the same (x or y)
-//   row, i.e. contiguous memory;
+// * lanes of a wavefront are 64 consecutive detector rows of one column and
+//   every thread traces R rows 64 apart.  For rotation about z this makes a
+//   wave sample consecutive z voxels of the same (x or y) row, contiguous
+//   memory;
+// * the slice loop is wave-uniform: it covers the union of the wave's ray
+//   supports (samples outside a ray's support clamp into the zero halo), so
+//   the slice base address lives in scalar registers and every tap is an
+//   SGPR-base load with a 32-bit per-lane offset;
+// * for a circular orbit the row step has no in-plane component, so the
+//   b-coordinate, its tap row and weight are shared by the thread's R rows
+//   ("column-coherent" angles, flagged per angle on the host);
 // * per-angle parameters are wave-uniform (scalar loads); workgroups are
 //   ordered row band -> angle -> column tile, and grouped so that each XCD
 //   keeps the same column range across consecutive angles (L2 reuse).
 #include "kk_internal.h"
 
 namespace {
+
+constexpr int kWave = 64;
+constexpr int kColsPerBlock = 4;  // one column per wavefront
 
 struct FpArgs {
   const float* tex;  // padded volume texture, element [0][0][0]
@@ -38,92 +48,98 @@ struct FpArgs {
   const float* R;  // mode 2: ray weights
   int64_t rs0, rs1, rs2;
   int32_t rows, cols, n_angles;
-  int32_t tile_rows, tile_cols, row_tiles, col_tiles;
+  int32_t row_tiles, col_tiles;
   int32_t mode;  // 0: out = p, 1: out += p, 2: out = (p - y) * R
   int32_t xcd_group;
   int64_t total_blocks;
 };
 
-template <bool PAIR>
-__device__ __forceinline__ float joseph_sample(const float* __restrict__ base,
-                                               int32_t sa, int32_t sb,
-                                               int32_t sc, float a1, float a2,
-                                               float b1, float b2, float nbf,
-                                               float ncf, int32_t i) {
-  const float fi = static_cast<float>(i);
-  const float xb = kk_clamp(fmaf(a1, fi, b1), -1.f, nbf);
-  const float xc = kk_clamp(fmaf(a2, fi, b2), -1.f, ncf);
-  const float jbf = floorf(xb);
-  const float jcf = floorf(xc);
-  const float fb = xb - jbf;
-  const float fc = xc - jcf;
-  const float* p = base + (i * sa + static_cast<int32_t>(jbf) * sb +
-                           static_cast<int32_t>(jcf) * sc);
-  float p00, p01, p10, p11;
-  if constexpr (PAIR) {
-    const kk_float2u lo = *reinterpret_cast<const kk_float2u*>(p);
-    const kk_float2u hi = *reinterpret_cast<const kk_float2u*>(p + sb);
-    p00 = lo.x;
-    p01 = lo.y;
-    p10 = hi.x;
-    p11 = hi.y;
-  } else {
-    p00 = p[0];
-    p01 = p[sc];
-    p10 = p[sb];
-    p11 = p[sb + sc];
-  }
-  const float v0 = fmaf(fc, p01 - p00, p00);
-  const float v1 = fmaf(fc, p11 - p10, p10);
-  return fmaf(fb, v1 - v0, v0);
-}
-
-template <bool PAIR>
-__device__ __forceinline__ float joseph_ray(const float* __restrict__ base,
-                                            int32_t sa, int32_t sb,
-                                            int32_t sc, float a1, float a2,
-                                            float b1, float b2, float nbf,
-                                            float ncf, int32_t i0,
-                                            int32_t i1) {
-  float acc0 = 0.f, acc1 = 0.f, acc2 = 0.f, acc3 = 0.f;
-  int32_t i = i0;
-  for (; i + 3 <= i1; i += 4) {
-    acc0 += joseph_sample<PAIR>(base, sa, sb, sc, a1, a2, b1, b2, nbf, ncf, i);
-    acc1 += joseph_sample<PAIR>(base, sa, sb, sc, a1, a2, b1, b2, nbf, ncf,
-                                i + 1);
-    acc2 += joseph_sample<PAIR>(base, sa, sb, sc, a1, a2, b1, b2, nbf, ncf,
-                                i + 2);
-    acc3 += joseph_sample<PAIR>(base, sa, sb, sc, a1, a2, b1, b2, nbf, ncf,
-                                i + 3);
-  }
-  for (; i <= i1; ++i)
-    acc0 += joseph_sample<PAIR>(base, sa, sb, sc, a1, a2, b1, b2, nbf, ncf, i);
-  return (acc0 + acc1) + (acc2 + acc3);
-}
-
 // Narrow [lo, hi] (slice indices, initially [0, n_a - 1]) to the slices i
-// with -1 < a*i + beta < n.  For slopes below 1e-3 the division is not
-// trusted; the ray is then only dropped when it stays outside over the
-// whole slice range.  Over-inclusion is harmless: clamped samples outside
-// the volume read the zero halo.
-__device__ __forceinline__ void clip_range(float a, float beta, float n,
+// with 0 < a*i + beta < n + 1 (padded coordinates).  For slopes below 1e-3
+// the division is not trusted; the ray is then only dropped when it stays
+// outside over the whole slice range.  Over-inclusion is harmless: clamped
+// samples outside the volume read the zero halo.
+__device__ __forceinline__ void clip_range(float a, float beta, float n1,
                                            float last, float& lo,
                                            float& hi) {
   if (fabsf(a) < 1e-3f) {
     const float e0 = beta, e1 = fmaf(a, last, beta);
-    if (fmaxf(e0, e1) <= -1.f || fminf(e0, e1) >= n) {
+    if (fmaxf(e0, e1) <= 0.f || fminf(e0, e1) >= n1) {
       lo = 3e38f;
       hi = -3e38f;
     }
     return;
   }
   const float inv = 1.f / a;
-  const float t0 = (-1.f - beta) * inv;
-  const float t1 = (n - beta) * inv;
+  const float t0 = (0.f - beta) * inv;
+  const float t1 = (n1 - beta) * inv;
   lo = fmaxf(lo, fminf(t0, t1));
   hi = fminf(hi, fmaxf(t0, t1));
 }
 
+// Bilinear combination of the four taps (v_c = lerp along c per b-row,
+// then lerp along b).
+__device__ __forceinline__ float bilerp(kk_float2u t0, kk_float2u t1,
+                                        float fb, float fc) {
+  const float v0 = fmaf(fc, t0.y - t0.x, t0.x);
+  const float v1 = fmaf(fc, t1.y - t1.x, t1.x);
+  return fmaf(fb, v1 - v0, v0);
+}
+
+// One tracing pass over slices [i0, i1] (wave-uniform) for R rays of a
+// thread.  COLC: a1/b1 are shared by the R rays.  PAIR: c is contiguous.
+template <int R, bool COLC, bool PAIR>
+__device__ __forceinline__ void trace(const kk::FpAngle& P,
+                                      const char* __restrict__ tex,
+                                      const float (&a1)[R],
+                                      const float (&b1)[R],
+                                      const float (&a2)[R],
+                                      const float (&b2)[R], int32_t i0,
+                                      int32_t i1, float (&acc)[R]) {
+  const float nb1 = static_cast<float>(P.nb + 1);
+  const float nc1 = static_cast<float>(P.nc + 1);
+  const uint32_t sab = P.sab, sbb = P.sbb, scb = P.scb;
+  // slice i of the volume is padded slice i + KK_HALO_LO
+  const char* slice = tex + static_cast<size_t>(i0 + kk::kHaloLo) * sab;
+  for (int32_t i = i0; i <= i1; ++i, slice += sab) {
+    const float fi = static_cast<float>(i);
+    const char* slice_b = slice + sbb;  // next b-row, uniform
+    float fbs = 0.f;
+    uint32_t offb = 0;
+    if constexpr (COLC) {
+      const float xb = kk_clamp(fmaf(a1[0], fi, b1[0]), 0.f, nb1);
+      fbs = __builtin_amdgcn_fractf(xb);
+      offb = __umul24(static_cast<uint32_t>(xb), sbb);
+    }
+#pragma unroll
+    for (int k = 0; k < R; ++k) {
+      float fb = fbs;
+      uint32_t ob = offb;
+      if constexpr (!COLC) {
+        const float xb = kk_clamp(fmaf(a1[k], fi, b1[k]), 0.f, nb1);
+        fb = __builtin_amdgcn_fractf(xb);
+        ob = __umul24(static_cast<uint32_t>(xb), sbb);
+      }
+      const float xc = kk_clamp(fmaf(a2[k], fi, b2[k]), 0.f, nc1);
+      const float fc = __builtin_amdgcn_fractf(xc);
+      const uint32_t jc = static_cast<uint32_t>(xc);
+      if constexpr (PAIR) {
+        const uint32_t off = ob + (jc << 2);
+        acc[k] += bilerp(kk_ld2(slice, off), kk_ld2(slice_b, off), fb, fc);
+      } else {
+        const uint32_t off = ob + __umul24(jc, scb);
+        kk_float2u t0, t1;
+        t0.x = kk_ld1(slice, off);
+        t0.y = kk_ld1(slice + scb, off);
+        t1.x = kk_ld1(slice_b, off);
+        t1.y = kk_ld1(slice_b + scb, off);
+        acc[k] += bilerp(t0, t1, fb, fc);
+      }
+    }
+  }
+}
+
+template <int R>
 __global__ __launch_bounds__(256) void cone_fp_kernel(
     const kk::FpAngle* __restrict__ tab, FpArgs A) {
   const int64_t bid =
@@ -132,53 +148,83 @@ __global__ __launch_bounds__(256) void cone_fp_kernel(
   const int64_t t = bid / A.col_tiles;
   const int32_t ang = static_cast<int32_t>(t % A.n_angles);
   const int32_t rt = static_cast<int32_t>(t / A.n_angles);
-  const int32_t row = rt * A.tile_rows + int32_t(threadIdx.x) % A.tile_rows;
-  const int32_t col = ct * A.tile_cols + int32_t(threadIdx.x) / A.tile_rows;
-  if (row >= A.rows || col >= A.cols) return;
+  const int32_t lane = int32_t(threadIdx.x) & (kWave - 1);
+  const int32_t col = ct * kColsPerBlock + int32_t(threadIdx.x) / kWave;
+  const int32_t row0 = rt * (kWave * R) + lane;
 
   const kk::FpAngle& P = tab[ang];
-  const float fr = static_cast<float>(row);
+  const float nb1 = static_cast<float>(P.nb + 1);
+  const float nc1 = static_cast<float>(P.nc + 1);
+  const float last = static_cast<float>(P.na - 1);
   const float fcl = static_cast<float>(col);
-  const float Da = fmaf(fr, P.va, fmaf(fcl, P.ua, P.qa));
-  const float Db = fmaf(fr, P.vb, fmaf(fcl, P.ub, P.qb));
-  const float Dc = fmaf(fr, P.vc, fmaf(fcl, P.uc, P.qc));
-  const float a1 = Db / Da;
-  const float a2 = Dc / Da;
-  const float nbf = static_cast<float>(P.nb);
-  const float ncf = static_cast<float>(P.nc);
 
-  const float b1 = fmaf(a1, P.sig, P.taub);
-  const float b2 = fmaf(a2, P.sig, P.tauc);
-  float val = 0.f;
-  // Non-finite slopes/intercepts (a ray parallel to the slices) contribute
-  // nothing; finite ones keep every clamped coordinate finite.
-  if (isfinite(a1) && isfinite(a2) && isfinite(b1) && isfinite(b2)) {
-    const float last = static_cast<float>(P.na - 1);
+  float a1[R], b1[R], a2[R], b2[R], wscale[R];
+  bool ok[R];
+  int32_t ilo = 0x7fffffff, ihi = -1;
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int32_t row = row0 + kWave * k;
+    const float fr = static_cast<float>(row);
+    const float Da = fmaf(fr, P.va, fmaf(fcl, P.ua, P.qa));
+    const float Db = fmaf(fr, P.vb, fmaf(fcl, P.ub, P.qb));
+    const float Dc = fmaf(fr, P.vc, fmaf(fcl, P.uc, P.qc));
+    a1[k] = Db / Da;
+    a2[k] = Dc / Da;
+    b1[k] = fmaf(a1[k], P.sig, P.taub);
+    b2[k] = fmaf(a2[k], P.sig, P.tauc);
+    // rays outside the detector, or parallel to the slices, contribute
+    // nothing: park them in the halo
+    ok[k] = row < A.rows && col < A.cols && isfinite(a1[k]) &&
+            isfinite(a2[k]) && isfinite(b1[k]) && isfinite(b2[k]);
+    wscale[k] = 0.f;
+    if (!ok[k]) {
+      a1[k] = a2[k] = 0.f;
+      b1[k] = b2[k] = -1.f;
+      continue;
+    }
+    wscale[k] = sqrtf(fmaf(a1[k] * a1[k], P.sc1,
+                           fmaf(a2[k] * a2[k], P.sc2, 1.f))) * P.oscale;
     float lo = 0.f, hi = last;
-    clip_range(a1, b1, nbf, last, lo, hi);
-    clip_range(a2, b2, ncf, last, lo, hi);
+    clip_range(a1[k], b1[k], nb1, last, lo, hi);
+    clip_range(a2[k], b2[k], nc1, last, lo, hi);
     if (lo <= hi + 2.f) {
-      const int32_t i0 = max(0, static_cast<int32_t>(floorf(lo)) - 1);
-      const int32_t i1 = min(P.na - 1, static_cast<int32_t>(ceilf(hi)) + 1);
-      const float* base = A.tex + kk::kHaloLo * (P.sa + P.sb + P.sc);
-      const float sum =
-          P.pair ? joseph_ray<true>(base, P.sa, P.sb, P.sc, a1, a2, b1, b2,
-                                    nbf, ncf, i0, i1)
-                 : joseph_ray<false>(base, P.sa, P.sb, P.sc, a1, a2, b1, b2,
-                                     nbf, ncf, i0, i1);
-      val = sum * sqrtf(fmaf(a1 * a1, P.sc1, fmaf(a2 * a2, P.sc2, 1.f))) *
-            P.oscale;
+      ilo = min(ilo, max(0, static_cast<int32_t>(floorf(lo)) - 1));
+      ihi = max(ihi, min(P.na - 1, static_cast<int32_t>(ceilf(hi)) + 1));
     }
   }
-  const int64_t o = ang * A.os0 + row * A.os1 + col * A.os2;
-  if (A.mode == 0) {
-    A.out[o] = val;
-  } else if (A.mode == 1) {
-    A.out[o] += val;
-  } else {
-    const float yv = A.y[ang * A.ys0 + row * A.ys1 + col * A.ys2];
-    const float rv = A.R[ang * A.rs0 + row * A.rs1 + col * A.rs2];
-    A.out[o] = (val - yv) * rv;
+  ilo = kk_wave_min(ilo);
+  ihi = kk_wave_max(ihi);
+
+  float acc[R];
+#pragma unroll
+  for (int k = 0; k < R; ++k) acc[k] = 0.f;
+  const char* tex = reinterpret_cast<const char*>(A.tex);
+  if (ilo <= ihi) {
+    if (P.pair) {
+      if (P.colc)
+        trace<R, true, true>(P, tex, a1, b1, a2, b2, ilo, ihi, acc);
+      else
+        trace<R, false, true>(P, tex, a1, b1, a2, b2, ilo, ihi, acc);
+    } else {
+      trace<R, false, false>(P, tex, a1, b1, a2, b2, ilo, ihi, acc);
+    }
+  }
+
+#pragma unroll
+  for (int k = 0; k < R; ++k) {
+    const int32_t row = row0 + kWave * k;
+    if (row >= A.rows || col >= A.cols) continue;
+    const float val = acc[k] * wscale[k];
+    const int64_t o = ang * A.os0 + row * A.os1 + col * A.os2;
+    if (A.mode == 0) {
+      A.out[o] = val;
+    } else if (A.mode == 1) {
+      A.out[o] += val;
+    } else {
+      const float yv = A.y[ang * A.ys0 + row * A.ys1 + col * A.ys2];
+      const float rv = A.R[ang * A.rs0 + row * A.rs1 + col * A.rs2];
+      A.out[o] = (val - yv) * rv;
+    }
   }
 }
 
@@ -189,8 +235,8 @@ namespace kk {
 int launch_cone_fp(const kk_plan* plan, const float* tex, float* out,
                    const int64_t os[3], int32_t rows, int32_t cols,
                    int32_t mode, const float* y, const int64_t ys[3],
-                   const float* R, const int64_t rs[3], int32_t tile_rows,
-                   hipStream_t stream) {
+                   const float* R, const int64_t rs[3],
+                   int32_t rows_per_thread, hipStream_t stream) {
   if (plan == nullptr || !plan->fp_set)
     return fail(KK_ERR_NOT_SET,
                 "Please set the projection geometry with `set_params()`.");
@@ -201,8 +247,8 @@ int launch_cone_fp(const kk_plan* plan, const float* tex, float* out,
   if (mode == 2 && (y == nullptr || R == nullptr || ys == nullptr ||
                     rs == nullptr))
     return fail(KK_ERR_INVALID, "kk_cone_fp: residual mode needs y and R");
-  if (tile_rows <= 0 || tile_rows > 256 || 256 % tile_rows != 0)
-    return fail(KK_ERR_INVALID, "kk_cone_fp: tile_rows must divide 256");
+  if (rows_per_thread != 1 && rows_per_thread != 2 && rows_per_thread != 4)
+    rows_per_thread = rows > 128 ? 4 : (rows > 64 ? 2 : 1);
   FpArgs A{};
   A.tex = tex;
   A.out = out;
@@ -222,17 +268,29 @@ int launch_cone_fp(const kk_plan* plan, const float* tex, float* out,
   A.rows = rows;
   A.cols = cols;
   A.n_angles = plan->fp_n;
-  A.tile_rows = tile_rows;
-  A.tile_cols = 256 / tile_rows;
-  A.row_tiles = (rows + A.tile_rows - 1) / A.tile_rows;
-  A.col_tiles = (cols + A.tile_cols - 1) / A.tile_cols;
+  A.row_tiles = (rows + kWave * rows_per_thread - 1) /
+                (kWave * rows_per_thread);
+  A.col_tiles = (cols + kColsPerBlock - 1) / kColsPerBlock;
   A.mode = mode;
   A.total_blocks = int64_t(A.row_tiles) * A.n_angles * A.col_tiles;
   if (A.total_blocks >= (int64_t(1) << 31))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_fp: grid too large");
   A.xcd_group = (A.col_tiles % kNumXcd == 0) ? A.col_tiles / kNumXcd : 1;
-  hipLaunchKernelGGL(cone_fp_kernel, dim3(unsigned(A.total_blocks)),
-                     dim3(256), 0, stream, plan->fp, A);
+  const dim3 grid(unsigned(A.total_blocks)), block(256);
+  switch (rows_per_thread) {
+    case 1:
+      hipLaunchKernelGGL(cone_fp_kernel<1>, grid, block, 0, stream, plan->fp,
+                         A);
+      break;
+    case 2:
+      hipLaunchKernelGGL(cone_fp_kernel<2>, grid, block, 0, stream, plan->fp,
+                         A);
+      break;
+    default:
+      hipLaunchKernelGGL(cone_fp_kernel<4>, grid, block, 0, stream, plan->fp,
+                         A);
+      break;
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "cone_fp_kernel");
   return KK_OK;
@@ -245,7 +303,7 @@ extern "C" int kk_cone_fp(const kk_plan* plan, const float* vol_tex,
                           int32_t cols, int32_t additive, void* stream) {
   return kk::launch_cone_fp(plan, vol_tex, proj, ps, rows, cols,
                             additive ? 1 : 0, nullptr, nullptr, nullptr,
-                            nullptr, 64, static_cast<hipStream_t>(stream));
+                            nullptr, 0, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
@@ -254,5 +312,5 @@ extern "C" int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
                                    float* out, const int64_t os[3],
                                    int32_t rows, int32_t cols, void* stream) {
   return kk::launch_cone_fp(plan, x_tex, out, os, rows, cols, 2, y, ys, R,
-                            rs, 64, static_cast<hipStream_t>(stream));
+                            rs, 0, static_cast<hipStream_t>(stream));
 }

@@ -25,29 +25,34 @@ constexpr int kNumXcd = 8;
 
 // Per-angle forward-projection record, float32, in the ray frame (a, b, c)
 // of that angle: a = ray (major) axis, b/c = the two in-plane axes, with
-// (a, b, c) = (x, y, z), (y, x, z) or (z, y, x) for a = x, y, z.
+// (a, b, c) = (x, y, z), (y, x, z) or (z, y, x) for a = x, y, z; b and c are
+// swapped when that makes c the texture's contiguous axis.
 struct alignas(16) FpAngle {
   float qa, qb, qc, _p0;    // centre of pixel (0,0) minus source
   float ua, ub, uc, _p1;    // column step U
   float va, vb, vc, _p2;    // row step V
-  float sig, taub, tauc, _p3;  // slice i <-> b-coordinate offsets (below)
+  float sig, taub, tauc, _p3;  // slice i <-> in-plane coordinate offsets
   float sc1, sc2, oscale, _p4; // sqrt(1 + a1^2 sc1 + a2^2 sc2) * oscale
   int32_t na, nb, nc, pair;    // extents along a, b, c; pair: c contiguous
-  int32_t sa, sb, sc, axis;    // texture strides along a, b, c; world axis a
-  int32_t _p5[4];
+  uint32_t sab, sbb, scb;      // texture strides along a, b, c in bytes
+  int32_t axis;                // world axis of a
+  int32_t colc;                // 1: V has no a/b part (a1, b1 row-free)
+  int32_t _p5[3];
 };
 // Ray through pixel (r, c): D = q + c*U + r*V (relative to the source),
-// a1 = D_b / D_a, a2 = D_c / D_a.  Array coordinate along b at slice i:
-// x_b(i) = a1 * (i + sig) + taub, with sig = 1/2 - n_a/2 - s_a and
-// taub = s_b + n_b/2 - 1/2 (both computed in float64 from the
-// volume-centred geometry, then rounded once).
+// a1 = D_b / D_a, a2 = D_c / D_a.  Padded texture coordinate along b at
+// slice i: x_b(i) = a1 * (i + sig) + taub, with sig = 1/2 - n_a/2 - s_a and
+// taub = s_b + n_b/2 - 1/2 + KK_HALO_LO (both computed in float64 from the
+// volume-centred geometry, then rounded once), so x_b = j + f reads padded
+// elements j and j + 1; 0 < x_b < n_b + 1 is the support of the volume.
 static_assert(sizeof(FpAngle) == 128, "FpAngle layout");
 
 // Per-angle backprojection record: affine forms of the voxel centre X
 // (voxel frame, world order).  u = nu(X)/den(X) and v = nv(X)/den(X) are
-// array coordinates on the detector (the texel-centre 1/2 already removed),
-// 1/den^2 is the adjoint weight.  fast != 0 when nu and den do not depend
-// on z (circular orbit about z): then u and 1/den are shared by a z-run.
+// padded texture coordinates on the detector (texel-centre 1/2 removed,
+// KK_HALO_LO added), 1/den^2 is the adjoint weight.  fast != 0 when nu and
+// den do not depend on z (circular orbit about z): then u and 1/den are
+// shared by a z-run.
 struct alignas(16) BpAngle {
   float nux, nuy, nuz, nuw;
   float nvx, nvy, nvz, nvw;
@@ -90,3 +95,24 @@ __device__ __forceinline__ float kk_clamp(float x, float lo, float hi) {
 
 // Two consecutive floats, 4-byte aligned: one global_load_dwordx2.
 typedef float kk_float2u __attribute__((ext_vector_type(2), aligned(4)));
+
+// Loads at a wave-uniform base plus a 32-bit per-lane byte offset: the
+// compiler emits the SGPR-base ("saddr") form of global loads, so no 64-bit
+// address arithmetic is spent per lane.
+__device__ __forceinline__ kk_float2u kk_ld2(const char* base, uint32_t off) {
+  return *reinterpret_cast<const kk_float2u*>(base + off);
+}
+__device__ __forceinline__ float kk_ld1(const char* base, uint32_t off) {
+  return *reinterpret_cast<const float*>(base + off);
+}
+
+__device__ __forceinline__ int32_t kk_wave_min(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = min(v, __shfl_xor(v, o));
+  return __builtin_amdgcn_readfirstlane(v);
+}
+__device__ __forceinline__ int32_t kk_wave_max(int32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return __builtin_amdgcn_readfirstlane(v);
+}

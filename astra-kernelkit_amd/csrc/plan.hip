@@ -123,8 +123,8 @@ static int build_fp_table(const double* params, int32_t n,
     return kk::fail(KK_ERR_UNSUPPORTED,
                     "volume too large for 32-bit texture indexing");
   const int64_t astride[3] = {text[1] * text[2], text[2], 1};
-  int32_t tstride[3];
-  for (int k = 0; k < 3; ++k) tstride[volume_axes[k]] = int32_t(astride[k]);
+  int64_t tstride[3];  // element stride of each world axis
+  for (int k = 0; k < 3; ++k) tstride[volume_axes[k]] = astride[k];
   if (axes == nullptr && (rows <= 0 || cols <= 0))
     return kk::fail(KK_ERR_INVALID,
                     "kk_set_params_fp: rows/cols needed to find ray axes");
@@ -204,19 +204,30 @@ static int build_fp_table(const double* params, int32_t n,
     e.vb = float(V[b]);
     e.vc = float(V[c]);
     e.sig = float(0.5 - 0.5 * shape[a] - s[a]);
-    e.taub = float(s[b] + 0.5 * shape[b] - 0.5);
-    e.tauc = float(s[c] + 0.5 * shape[c] - 0.5);
+    e.taub = float(s[b] + 0.5 * shape[b] - 0.5 + kk::kHaloLo);
+    e.tauc = float(s[c] + 0.5 * shape[c] - 0.5 + kk::kHaloLo);
     e.sc1 = float(sc1);
     e.sc2 = float(sc2);
     e.oscale = float(osc[m][2]);
     e.na = shape[a];
     e.nb = shape[b];
     e.nc = shape[c];
-    e.sa = tstride[a];
-    e.sb = tstride[b];
-    e.sc = tstride[c];
-    e.pair = (e.sc == 1) ? 1 : 0;
+    // in-slice byte offsets (j_b * sbb + j_c * scb, j <= n + 2) must fit
+    // 32 bits, and sbb/scb 24 bits (v_mul_u32_u24)
+    const int64_t sbb = 4 * tstride[b], scb = 4 * tstride[c];
+    const int64_t span = (shape[b] + 2) * sbb + (shape[c] + 2) * scb + 8;
+    if (span >= (int64_t(1) << 32) || sbb >= (int64_t(1) << 24) ||
+        scb >= (int64_t(1) << 24))
+      return kk::fail(KK_ERR_UNSUPPORTED,
+                      "volume too large for 32-bit in-slice offsets along "
+                      "ray axis %d",
+                      m);
+    e.sab = uint32_t(4 * tstride[a]);
+    e.sbb = uint32_t(sbb);
+    e.scb = uint32_t(scb);
+    e.pair = (tstride[c] == 1) ? 1 : 0;
     e.axis = m;
+    e.colc = (V[a] == 0.0 && V[b] == 0.0) ? 1 : 0;
   }
   return KK_OK;
 }
@@ -252,12 +263,14 @@ static int build_bp_table(const double* params, int32_t n,
   tab.assign(n, kk::BpAngle{});
   for (int32_t k = 0; k < n; ++k) {
     const double* p = params + 12 * int64_t(k);
-    // numerators minus den/2: texture coordinate -> array coordinate
+    // numerators shifted by (halo - 1/2) * den: texture coordinate ->
+    // padded array coordinate
     double nu[4], nv[4], de[4];
+    const double shift = kk::kHaloLo - 0.5;
     for (int t = 0; t < 4; ++t) {
       de[t] = p[8 + t];
-      nu[t] = p[0 + t] - 0.5 * de[t];
-      nv[t] = p[4 + t] - 0.5 * de[t];
+      nu[t] = p[0 + t] + shift * de[t];
+      nv[t] = p[4 + t] + shift * de[t];
     }
     for (int t = 0; t < 12; ++t)
       if (!std::isfinite(p[t]))

@@ -229,8 +229,8 @@ FP_ANGLE_DTYPE = np.dtype([
     ("sig", "<f4"), ("taub", "<f4"), ("tauc", "<f4"), ("_p3", "<f4"),
     ("sc1", "<f4"), ("sc2", "<f4"), ("oscale", "<f4"), ("_p4", "<f4"),
     ("na", "<i4"), ("nb", "<i4"), ("nc", "<i4"), ("pair", "<i4"),
-    ("sa", "<i4"), ("sb", "<i4"), ("sc", "<i4"), ("axis", "<i4"),
-    ("_p5", "<i4", (4,))])
+    ("sab", "<u4"), ("sbb", "<u4"), ("scb", "<u4"), ("axis", "<i4"),
+    ("colc", "<i4"), ("_p5", "<i4", (3,))])
 BP_ANGLE_DTYPE = np.dtype([
     ("nux", "<f4"), ("nuy", "<f4"), ("nuz", "<f4"), ("nuw", "<f4"),
     ("nvx", "<f4"), ("nvy", "<f4"), ("nvz", "<f4"), ("nvw", "<f4"),

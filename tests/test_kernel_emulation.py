@@ -27,7 +27,10 @@ def _texture(arr):
 
 
 def emulate_fp(vol_array, table, rows, cols):
-    """cone_fp_kernel in float32 numpy: returns proj[angle, row, col]."""
+    """cone_fp_kernel in float32 numpy: returns proj[angle, row, col].
+
+    Coordinates are padded-texture coordinates (halo folded into the
+    intercepts), clamped to [0, n + 1]; slice i is padded slice i + 1."""
     tex = _texture(vol_array).ravel()
     n = len(table)
     out = np.zeros((n, rows, cols), f32)
@@ -43,17 +46,18 @@ def emulate_fp(vol_array, table, rows, cols):
         b1 = (a1 * f32(P["sig"]) + f32(P["taub"])).astype(f32)
         b2 = (a2 * f32(P["sig"]) + f32(P["tauc"])).astype(f32)
         na, nb, nc = int(P["na"]), int(P["nb"]), int(P["nc"])
-        sa, sb, sc = int(P["sa"]), int(P["sb"]), int(P["sc"])
-        base = _native.HALO_LO * (sa + sb + sc)
+        sab, sbb, scb = int(P["sab"]), int(P["sbb"]), int(P["scb"])
+        assert sab % 4 == 0 and sbb % 4 == 0 and scb % 4 == 0
+        sa, sb, sc = sab // 4, sbb // 4, scb // 4
         acc = np.zeros((rows, cols), f32)
         for i in range(na):  # the kernel's range clipping only skips zeros
-            xb = np.clip(a1 * f32(i) + b1, -1, nb).astype(f32)
-            xc = np.clip(a2 * f32(i) + b2, -1, nc).astype(f32)
+            xb = np.clip(a1 * f32(i) + b1, 0, nb + 1).astype(f32)
+            xc = np.clip(a2 * f32(i) + b2, 0, nc + 1).astype(f32)
             jb = np.floor(xb)
             jc = np.floor(xc)
             fb = xb - jb
             fc = xc - jc
-            idx = base + i * sa + jb.astype(np.int64) * sb + \
+            idx = (i + _native.HALO_LO) * sa + jb.astype(np.int64) * sb + \
                 jc.astype(np.int64) * sc
             p00, p01 = tex[idx], tex[idx + sc]
             p10, p11 = tex[idx + sb], tex[idx + sb + sc]
@@ -67,7 +71,9 @@ def emulate_fp(vol_array, table, rows, cols):
 
 
 def emulate_bp(proj, table, shape, scale):
-    """cone_bp_kernel in float32 numpy (both paths): returns vol[x, y, z]."""
+    """cone_bp_kernel in float32 numpy (both paths): returns vol[x, y, z].
+
+    The forms give padded texture coordinates, clamped to [0, n + 1]."""
     n, rows, cols = proj.shape
     tex = _texture(proj)
     X, Y, Z = np.meshgrid(*[np.arange(s, dtype=f32) - f32(0.5 * s)
@@ -82,12 +88,12 @@ def emulate_bp(proj, table, shape, scale):
         nu = nu + Z * f32(P["nuz"])
         nv = nv + Z * f32(P["nvz"])
         r = (f32(1) / den).astype(f32)
-        u = np.clip(nu * r, -1, cols)
-        v = np.clip(nv * r, -1, rows)
+        u = np.clip(nu * r, 0, cols + 1)
+        v = np.clip(nv * r, 0, rows + 1)
         uf, vf = np.floor(u), np.floor(v)
         fu, fv = u - uf, v - vf
-        ui = uf.astype(np.int64) + lo
-        vi = vf.astype(np.int64) + lo
+        ui = uf.astype(np.int64)
+        vi = vf.astype(np.int64)
         img = tex[k + lo]
         h0 = fu * (img[vi, ui + 1] - img[vi, ui]) + img[vi, ui]
         h1 = fu * (img[vi + 1, ui + 1] - img[vi + 1, ui]) + img[vi + 1, ui]
@@ -147,7 +153,7 @@ def test_full_config_fp_table_precision():
     fr = oracle.normalized_frames(sub, vg)
     for k, P in enumerate(t):
         a, b, c = {0: (0, 1, 2), 1: (1, 0, 2), 2: (2, 1, 0)}[int(P["axis"])]
-        if int(P["sb"]) == 1:
+        if int(P["sbb"]) == 4:
             b, c = c, b
         s, d0 = fr["s"][k], fr["d0"][k]
         U, V = fr["U"][k], fr["V"][k]
@@ -155,7 +161,7 @@ def test_full_config_fp_table_precision():
             D = d0 + (col + 0.5) * U + (r + 0.5) * V
             a1 = (D[b] - s[b]) / (D[a] - s[a])
             for i in (0, 255, 511):
-                exact = s[b] + a1 * (i - 255.5 - s[a]) + 255.5
+                exact = s[b] + a1 * (i - 255.5 - s[a]) + 255.5 + 1.0
                 Df = [np.float32(P[f"q{x}"]) + np.float32(col) *
                       np.float32(P[f"u{x}"]) + np.float32(r) *
                       np.float32(P[f"v{x}"]) for x in "abc"]
