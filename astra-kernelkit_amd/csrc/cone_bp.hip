@@ -16,9 +16,10 @@
 //   once per (x, y, angle) and shared by the K voxels; rows advance by a
 //   constant step.  Other geometries take the general per-voxel path (a
 //   wave-uniform branch per angle);
-// * projections are read from a zero-halo texture [angle][row][col]: a
-//   sample is a clamp, two 8-byte loads (column pairs of two rows) and
-//   three lerps, with no bounds tests;
+// * projections are read from a zero-halo texture of (value, next-row)
+//   float2 pairs (kk_internal.h): a sample is a clamp, ONE 16-byte load for
+//   the four taps and three lerps, with no bounds tests (the
+//   texture-address unit, not the ALU, limits a gather kernel);
 // * lanes of a wavefront are a 16 x 4 (x, y) patch, which projects onto a
 //   compact detector patch; workgroups are ordered z-chunk -> y -> x and
 //   grouped per XCD so each L2 sees a contiguous part of the (x, y) plane.
@@ -26,10 +27,12 @@
 
 namespace {
 
+constexpr int kMaxFrames = 3;
+
 struct BpArgs {
-  const float* tex;  // padded projection texture, element [0][0][0]
-  int64_t tas;       // texture stride of one angle
-  int32_t trs, tcs;  // texture strides of one row / one column
+  const char* tex;   // projection texture (byte base)
+  uint32_t tab_b;    // bytes per angle image ((rows+3) * (cols+3) * 8)
+  uint32_t trb;      // bytes per texture row ((cols+3) * 8)
   int32_t nx, ny, nz, n_angles, rows, cols;
   float* out;
   int64_t vs0, vs1, vs2;  // output world strides
@@ -38,33 +41,28 @@ struct BpArgs {
   int32_t tiles_x, tiles_y, tiles_z;
   int32_t xcd_group;
   int64_t total_blocks;
-  // mode 2: x <- clip(mask * (x - C * b)), optionally mirrored into xtex
+  // mode 2: x <- clip(mask * (x - C * b)), mirrored into the FP textures
   const float* C;
   int64_t cs0, cs1, cs2;
   const float* mask;
   int64_t ms0, ms1, ms2;
   float lo, hi;
   int32_t clip_lo, clip_hi;
-  float* xtex;  // FP texture of x at voxel (0,0,0) (interior), or null
-  int64_t xts0, xts1, xts2;  // its world strides
+  float* xtex;
+  int32_t n_frames;
+  int64_t fr_origin[kMaxFrames], fr_s0[kMaxFrames], fr_s1[kMaxFrames],
+      fr_s2[kMaxFrames], fr_dual[kMaxFrames];
 };
 
-// Two horizontally adjacent detector taps (columns u0, u0+1) at a
-// wave-uniform base plus a per-lane byte offset.
-template <bool PAIR>
-__device__ __forceinline__ kk_float2u col_pair(const char* base, uint32_t off,
-                                               uint32_t tcb) {
-  if constexpr (PAIR) {
-    return kk_ld2(base, off);
-  } else {
-    kk_float2u t;
-    t.x = kk_ld1(base, off);
-    t.y = kk_ld1(base + tcb, off);
-    return t;
-  }
+// Bilinear value from the 16-byte tap record (p[r][c], p[r+1][c],
+// p[r][c+1], p[r+1][c+1]): lerp along rows (fv), then columns (fu).
+__device__ __forceinline__ float bilerp(kk_float4u t, float fu, float fv) {
+  const float l0 = fmaf(fv, t.y - t.x, t.x);
+  const float l1 = fmaf(fv, t.w - t.z, t.z);
+  return fmaf(fu, l1 - l0, l0);
 }
 
-template <int K, bool PAIR>
+template <int K>
 __global__ __launch_bounds__(256) void cone_bp_kernel(
     const kk::BpAngle* __restrict__ tab, BpArgs A) {
   const int64_t bid =
@@ -83,19 +81,15 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
   const float Z0 = static_cast<float>(iz0) - 0.5f * A.nz + 0.5f;
   const float rmax = static_cast<float>(A.rows + 1);
   const float cmax = static_cast<float>(A.cols + 1);
-  const uint32_t trb = 4u * static_cast<uint32_t>(A.trs);
-  const uint32_t tcb = PAIR ? 4u : 4u * static_cast<uint32_t>(A.tcs);
+  const uint32_t trb = A.trb;
 
   float acc[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0.f;
 
-  // padded image of angle 0; the halo offset is folded into the forms
-  const char* img = reinterpret_cast<const char*>(A.tex + A.tas);
-  const size_t tasb = 4 * static_cast<size_t>(A.tas);
-  for (int32_t a = 0; a < A.n_angles; ++a, img += tasb) {
+  const char* img = A.tex;
+  for (int32_t a = 0; a < A.n_angles; ++a, img += A.tab_b) {
     const kk::BpAngle& P = tab[a];
-    const char* img_r = img + trb;  // next detector row, uniform
     const float den = fmaf(Y, P.dy, fmaf(X, P.dx, P.dw));
     const float nu = fmaf(Y, P.nuy, fmaf(X, P.nux, P.nuw));
     const float nv = fmaf(Y, P.nvy, fmaf(X, P.nvx, P.nvw));
@@ -110,7 +104,7 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
       const float w = r * r;
       const float u = kk_clamp(un, 0.f, cmax);
       const float fu = __builtin_amdgcn_fractf(u);
-      const uint32_t offu = __umul24(static_cast<uint32_t>(u), tcb);
+      const uint32_t offu = static_cast<uint32_t>(u) << 3;
       uint32_t off[K];
       float fv[K];
 #pragma unroll
@@ -120,18 +114,12 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
         fv[k] = __builtin_amdgcn_fractf(v);
         off[k] = __umul24(static_cast<uint32_t>(v), trb) + offu;
       }
-      kk_float2u t0[K], t1[K];
+      kk_float4u tp[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        t0[k] = col_pair<PAIR>(img, off[k], tcb);
-        t1[k] = col_pair<PAIR>(img_r, off[k], tcb);
-      }
+      for (int k = 0; k < K; ++k) tp[k] = kk_ld4(img, off[k]);
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const float h0 = fmaf(fu, t0[k].y - t0[k].x, t0[k].x);
-        const float h1 = fmaf(fu, t1[k].y - t1[k].x, t1[k].x);
-        acc[k] = fmaf(w, fmaf(fv[k], h1 - h0, h0), acc[k]);
-      }
+      for (int k = 0; k < K; ++k)
+        acc[k] = fmaf(w, bilerp(tp[k], fu, fv[k]), acc[k]);
     } else {
       uint32_t off[K];
       float fu[K], fv[K], w[K];
@@ -147,21 +135,15 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
         fu[k] = __builtin_amdgcn_fractf(u);
         fv[k] = __builtin_amdgcn_fractf(v);
         off[k] = __umul24(static_cast<uint32_t>(v), trb) +
-                 __umul24(static_cast<uint32_t>(u), tcb);
+                 (static_cast<uint32_t>(u) << 3);
         w[k] = ok ? r * r : 0.f;
       }
-      kk_float2u t0[K], t1[K];
+      kk_float4u tp[K];
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        t0[k] = col_pair<PAIR>(img, off[k], tcb);
-        t1[k] = col_pair<PAIR>(img_r, off[k], tcb);
-      }
+      for (int k = 0; k < K; ++k) tp[k] = kk_ld4(img, off[k]);
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const float h0 = fmaf(fu[k], t0[k].y - t0[k].x, t0[k].x);
-        const float h1 = fmaf(fu[k], t1[k].y - t1[k].x, t1[k].x);
-        acc[k] = fmaf(w[k], fmaf(fv[k], h1 - h0, h0), acc[k]);
-      }
+      for (int k = 0; k < K; ++k)
+        acc[k] = fmaf(w[k], bilerp(tp[k], fu[k], fv[k]), acc[k]);
     }
   }
 
@@ -182,7 +164,12 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
       if (A.clip_lo) x = fmaxf(x, A.lo);
       if (A.clip_hi) x = fminf(x, A.hi);
       A.out[o] = x;
-      if (A.xtex) A.xtex[ix * A.xts0 + iy * A.xts1 + iz * A.xts2] = x;
+      for (int f = 0; f < A.n_frames; ++f) {
+        const int64_t q = A.fr_origin[f] + ix * A.fr_s0[f] +
+                          iy * A.fr_s1[f] + iz * A.fr_s2[f];
+        A.xtex[q] = x;
+        A.xtex[q + A.fr_dual[f]] = x;
+      }
     }
   }
 }
@@ -198,20 +185,19 @@ struct BpEpilogue {
   const int64_t* ms = nullptr;
   float lo = 0.f, hi = 0.f;
   int32_t clip_lo = 0, clip_hi = 0;
-  float* xtex = nullptr;
-  const int64_t* xts = nullptr;
+  float* xtex = nullptr;  // FP frame textures of x (layout kk_internal.h)
+  int32_t xtex_mask = 0;
 };
 
-int launch_cone_bp(const kk_plan* plan, const float* tex,
-                   const int32_t pax[3], int32_t n_angles, int32_t rows,
-                   int32_t cols, float* vol, const int32_t shape[3],
-                   const int64_t vs[3], float scale, int32_t mode,
-                   const BpEpilogue& ep, int32_t zrun, hipStream_t stream) {
+int launch_cone_bp(const kk_plan* plan, const float* tex, int32_t n_angles,
+                   int32_t rows, int32_t cols, float* vol,
+                   const int32_t shape[3], const int64_t vs[3], float scale,
+                   int32_t mode, const BpEpilogue& ep, int32_t zrun,
+                   hipStream_t stream) {
   if (plan == nullptr || !plan->bp_set)
     return fail(KK_ERR_NOT_SET,
                 "Please set the parameters with `set_params()`.");
-  if (tex == nullptr || vol == nullptr || shape == nullptr || vs == nullptr ||
-      pax == nullptr)
+  if (tex == nullptr || vol == nullptr || shape == nullptr || vs == nullptr)
     return fail(KK_ERR_INVALID, "kk_cone_bp: null argument");
   if (n_angles != plan->bp_n)
     return fail(KK_ERR_SHAPE,
@@ -220,35 +206,16 @@ int launch_cone_bp(const kk_plan* plan, const float* tex,
   if (rows <= 0 || cols <= 0 || shape[0] <= 0 || shape[1] <= 0 ||
       shape[2] <= 0)
     return fail(KK_ERR_INVALID, "kk_cone_bp: bad sizes");
-  int seen = 0;
-  for (int k = 0; k < 3; ++k)
-    if (pax[k] >= 0 && pax[k] < 3) seen |= 1 << pax[k];
-  if (seen != 7)
-    return fail(KK_ERR_INVALID,
-                "projection_axes must be a permutation of (0, 1, 2)");
-  // texture = padded projection array in its own axis order
-  const int64_t logical[3] = {n_angles, rows, cols};
-  int64_t text[3];
-  for (int k = 0; k < 3; ++k) text[k] = logical[pax[k]] + kHalo;
-  const int64_t astride[3] = {text[1] * text[2], text[2], 1};
-  int64_t lstride[3];
-  for (int k = 0; k < 3; ++k) lstride[pax[k]] = astride[k];
-  // in-image byte offsets (row <= rows + 2, col <= cols + 2) must fit 32
-  // bits and the row/column byte strides 24 bits (v_mul_u32_u24)
-  if (4 * ((rows + 2) * lstride[1] + (cols + 2) * lstride[2] + 2) >=
-          (int64_t(1) << 32) ||
-      4 * lstride[1] >= (int64_t(1) << 24) ||
-      4 * lstride[2] >= (int64_t(1) << 24))
-    return fail(KK_ERR_UNSUPPORTED,
-                "kk_cone_bp: projection layout too large for 32-bit "
-                "in-image offsets");
+  const int64_t trb = int64_t(cols + kHalo) * 8;
+  const int64_t tab_b = int64_t(rows + kHalo) * trb;
+  if (tab_b >= (int64_t(1) << 32) || trb >= (int64_t(1) << 24))
+    return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: detector too large");
   if (mode == 2 && (ep.C == nullptr || ep.cs == nullptr))
     return fail(KK_ERR_INVALID, "kk_cone_bp: SIRT update needs C");
   BpArgs A{};
-  A.tex = tex;
-  A.tas = lstride[0];
-  A.trs = int32_t(lstride[1]);
-  A.tcs = int32_t(lstride[2]);
+  A.tex = reinterpret_cast<const char*>(tex);
+  A.tab_b = uint32_t(tab_b);
+  A.trb = uint32_t(trb);
   A.nx = shape[0];
   A.ny = shape[1];
   A.nz = shape[2];
@@ -278,11 +245,25 @@ int launch_cone_bp(const kk_plan* plan, const float* tex,
     A.clip_hi = ep.clip_hi;
     A.xtex = ep.xtex;
     if (ep.xtex) {
-      if (ep.xts == nullptr)
-        return fail(KK_ERR_INVALID, "kk_cone_bp: texture strides missing");
-      A.xts0 = ep.xts[0];
-      A.xts1 = ep.xts[1];
-      A.xts2 = ep.xts[2];
+      // voxel (x, y, z) in frame a: element (c_a, c_b + 1, c_c + 1)
+      // component 0 and element (c_a, c_b, c_c + 1) component 1
+      int64_t off = 0;  // floats
+      for (int a = 0; a < 3; ++a) {
+        if (!(ep.xtex_mask & (1 << a))) continue;
+        const int b = kFrameB[a], c = kFrameC[a];
+        const int64_t tcn = shape[c] + kHalo, tbn = shape[b] + kHalo;
+        int64_t ws[3];
+        ws[a] = 2 * tbn * tcn;
+        ws[b] = 2 * tcn;
+        ws[c] = 2;
+        const int f = A.n_frames++;
+        A.fr_origin[f] = off + 2 * (kHaloLo * tcn + kHaloLo);
+        A.fr_s0[f] = ws[0];
+        A.fr_s1[f] = ws[1];
+        A.fr_s2[f] = ws[2];
+        A.fr_dual[f] = 1 - 2 * tcn;
+        off += 2 * frame_elems(shape, a);
+      }
     }
   }
   A.tiles_x = (A.nx + 15) / 16;
@@ -294,28 +275,22 @@ int launch_cone_bp(const kk_plan* plan, const float* tex,
   const int32_t plane = A.tiles_x * A.tiles_y;
   A.xcd_group = (plane % kNumXcd == 0) ? plane / kNumXcd : 1;
   const dim3 grid(unsigned(A.total_blocks)), block(256);
-  const bool pair = (A.tcs == 1);
-#define KK_BP_LAUNCH(KV)                                                   \
-  if (pair)                                                                \
-    hipLaunchKernelGGL((cone_bp_kernel<KV, true>), grid, block, 0, stream, \
-                       plan->bp, A);                                       \
-  else                                                                     \
-    hipLaunchKernelGGL((cone_bp_kernel<KV, false>), grid, block, 0, stream,\
-                       plan->bp, A);
   switch (zrun) {
     case 4:
-      KK_BP_LAUNCH(4);
+      hipLaunchKernelGGL(cone_bp_kernel<4>, grid, block, 0, stream, plan->bp,
+                         A);
       break;
     case 8:
-      KK_BP_LAUNCH(8);
+      hipLaunchKernelGGL(cone_bp_kernel<8>, grid, block, 0, stream, plan->bp,
+                         A);
       break;
     case 16:
-      KK_BP_LAUNCH(16);
+      hipLaunchKernelGGL(cone_bp_kernel<16>, grid, block, 0, stream,
+                         plan->bp, A);
       break;
     default:
       return fail(KK_ERR_INVALID, "kk_cone_bp: z-run must be 4, 8 or 16");
   }
-#undef KK_BP_LAUNCH
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "cone_bp_kernel");
   return KK_OK;
@@ -324,27 +299,26 @@ int launch_cone_bp(const kk_plan* plan, const float* tex,
 }  // namespace kk
 
 extern "C" int kk_cone_bp(const kk_plan* plan, const float* proj_tex,
-                          const int32_t pax[3], int32_t n_angles,
-                          int32_t rows, int32_t cols, float* vol,
-                          const int32_t shape[3], const int64_t vs[3],
-                          float output_scale, int32_t additive,
-                          void* stream) {
+                          int32_t n_angles, int32_t rows, int32_t cols,
+                          float* vol, const int32_t shape[3],
+                          const int64_t vs[3], float output_scale,
+                          int32_t additive, void* stream) {
   kk::BpEpilogue ep;
-  return kk::launch_cone_bp(plan, proj_tex, pax, n_angles, rows, cols, vol,
-                            shape, vs, output_scale, additive ? 1 : 0, ep, 8,
+  return kk::launch_cone_bp(plan, proj_tex, n_angles, rows, cols, vol, shape,
+                            vs, output_scale, additive ? 1 : 0, ep, 8,
                             static_cast<hipStream_t>(stream));
 }
 
 extern "C" int kk_sirt_bp_update(const kk_plan* plan, const float* res_tex,
-                                 const int32_t pax[3], int32_t n_angles,
-                                 int32_t rows, int32_t cols, float* x,
-                                 const int32_t shape[3],
-                                 const int64_t xs[3], const float* C,
-                                 const int64_t cs[3], const float* mask,
-                                 const int64_t ms[3], float lo, float hi,
-                                 int32_t clip_lo, int32_t clip_hi,
-                                 float* x_tex, const int64_t xts[3],
-                                 float output_scale, void* stream) {
+                                 int32_t n_angles, int32_t rows,
+                                 int32_t cols, float* x,
+                                 const int32_t shape[3], const int64_t xs[3],
+                                 const float* C, const int64_t cs[3],
+                                 const float* mask, const int64_t ms[3],
+                                 float lo, float hi, int32_t clip_lo,
+                                 int32_t clip_hi, float* x_tex,
+                                 int32_t x_tex_mask, float output_scale,
+                                 void* stream) {
   kk::BpEpilogue ep;
   ep.C = C;
   ep.cs = cs;
@@ -352,13 +326,15 @@ extern "C" int kk_sirt_bp_update(const kk_plan* plan, const float* res_tex,
   ep.ms = ms;
   if (mask != nullptr && ms == nullptr)
     return kk::fail(KK_ERR_INVALID, "kk_sirt_bp_update: mask strides");
+  if (x_tex != nullptr && (x_tex_mask <= 0 || x_tex_mask > 7))
+    return kk::fail(KK_ERR_INVALID, "kk_sirt_bp_update: texture mask");
   ep.lo = lo;
   ep.hi = hi;
   ep.clip_lo = clip_lo;
   ep.clip_hi = clip_hi;
   ep.xtex = x_tex;
-  ep.xts = xts;
-  return kk::launch_cone_bp(plan, res_tex, pax, n_angles, rows, cols, x,
-                            shape, xs, output_scale, 2, ep, 8,
+  ep.xtex_mask = x_tex_mask;
+  return kk::launch_cone_bp(plan, res_tex, n_angles, rows, cols, x, shape,
+                            xs, output_scale, 2, ep, 8,
                             static_cast<hipStream_t>(stream));
 }

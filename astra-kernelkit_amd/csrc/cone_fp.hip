@@ -11,10 +11,12 @@
 // * one launch covers all angles and all slices (the reference issues one
 //   launch per angle group and 16-slice chunk and accumulates in global
 //   memory); each output pixel is written exactly once;
-// * the volume is read from a zero-halo texture, so a sample is: one FMA per
-//   coordinate, a clamp into the halo, v_cvt_u32 + v_fract for the tap index
-//   and weight, and four taps with no bounds tests; when the c axis is the
-//   texture's contiguous axis the two c-neighbours are one 8-byte load;
+// * the volume is read from slice-major frame textures of (value,
+//   b-neighbour) float2 pairs with a zero halo (kk_internal.h), so a sample
+//   is: one FMA per coordinate, a clamp into the halo, v_cvt_u32 + v_fract
+//   for the tap index and weight, and ONE 16-byte load for all four taps
+//   (the texture-address unit, not the ALU, limits a gather kernel: one
+//   64-lane load instruction costs it ~16 cycles whatever its width);
 // * sample coordinates are evaluated directly, x_b(i) = a1*i + beta, instead
 //   of the reference's running sums, so the per-sample error does not grow
 //   along the ray;
@@ -40,9 +42,10 @@ constexpr int kWave = 64;
 constexpr int kColsPerBlock = 4;  // one column per wavefront
 
 struct FpArgs {
-  const float* tex;  // padded volume texture, element [0][0][0]
+  const char* tex;  // frame textures (byte base)
   float* out;
   int64_t os0, os1, os2;  // output strides: angle, row, col
+  int64_t out_dual;       // if nonzero: also write out[o + out_dual]
   const float* y;         // mode 2: measured projections
   int64_t ys0, ys1, ys2;
   const float* R;  // mode 2: ray weights
@@ -77,20 +80,19 @@ __device__ __forceinline__ void clip_range(float a, float beta, float n1,
   hi = fminf(hi, fmaxf(t0, t1));
 }
 
-// Bilinear combination of the four taps (v_c = lerp along c per b-row,
-// then lerp along b).
-__device__ __forceinline__ float bilerp(kk_float2u t0, kk_float2u t1,
-                                        float fb, float fc) {
-  const float v0 = fmaf(fc, t0.y - t0.x, t0.x);
-  const float v1 = fmaf(fc, t1.y - t1.x, t1.x);
-  return fmaf(fb, v1 - v0, v0);
+// Bilinear value from the 16-byte tap record (v[b][c], v[b+1][c],
+// v[b][c+1], v[b+1][c+1]): lerp along b, then along c.
+__device__ __forceinline__ float bilerp(kk_float4u t, float fb, float fc) {
+  const float l0 = fmaf(fb, t.y - t.x, t.x);
+  const float l1 = fmaf(fb, t.w - t.z, t.z);
+  return fmaf(fc, l1 - l0, l0);
 }
 
 // One tracing pass over slices [i0, i1] (wave-uniform) for R rays of a
-// thread.  COLC: a1/b1 are shared by the R rays.  PAIR: c is contiguous.
-template <int R, bool COLC, bool PAIR>
+// thread.  COLC: a1/b1 are shared by the R rays.
+template <int R, bool COLC>
 __device__ __forceinline__ void trace(const kk::FpAngle& P,
-                                      const char* __restrict__ tex,
+                                      const char* __restrict__ frame,
                                       const float (&a1)[R],
                                       const float (&b1)[R],
                                       const float (&a2)[R],
@@ -98,12 +100,10 @@ __device__ __forceinline__ void trace(const kk::FpAngle& P,
                                       int32_t i1, float (&acc)[R]) {
   const float nb1 = static_cast<float>(P.nb + 1);
   const float nc1 = static_cast<float>(P.nc + 1);
-  const uint32_t sab = P.sab, sbb = P.sbb, scb = P.scb;
-  // slice i of the volume is padded slice i + KK_HALO_LO
-  const char* slice = tex + static_cast<size_t>(i0 + kk::kHaloLo) * sab;
+  const uint32_t sab = P.sab, sbb = P.sbb;
+  const char* slice = frame + static_cast<size_t>(i0) * sab;
   for (int32_t i = i0; i <= i1; ++i, slice += sab) {
     const float fi = static_cast<float>(i);
-    const char* slice_b = slice + sbb;  // next b-row, uniform
     float fbs = 0.f;
     uint32_t offb = 0;
     if constexpr (COLC) {
@@ -122,19 +122,8 @@ __device__ __forceinline__ void trace(const kk::FpAngle& P,
       }
       const float xc = kk_clamp(fmaf(a2[k], fi, b2[k]), 0.f, nc1);
       const float fc = __builtin_amdgcn_fractf(xc);
-      const uint32_t jc = static_cast<uint32_t>(xc);
-      if constexpr (PAIR) {
-        const uint32_t off = ob + (jc << 2);
-        acc[k] += bilerp(kk_ld2(slice, off), kk_ld2(slice_b, off), fb, fc);
-      } else {
-        const uint32_t off = ob + __umul24(jc, scb);
-        kk_float2u t0, t1;
-        t0.x = kk_ld1(slice, off);
-        t0.y = kk_ld1(slice + scb, off);
-        t1.x = kk_ld1(slice_b, off);
-        t1.y = kk_ld1(slice_b + scb, off);
-        acc[k] += bilerp(t0, t1, fb, fc);
-      }
+      const uint32_t off = ob + (static_cast<uint32_t>(xc) << 3);
+      acc[k] += bilerp(kk_ld4(slice, off), fb, fc);
     }
   }
 }
@@ -159,7 +148,6 @@ __global__ __launch_bounds__(256) void cone_fp_kernel(
   const float fcl = static_cast<float>(col);
 
   float a1[R], b1[R], a2[R], b2[R], wscale[R];
-  bool ok[R];
   int32_t ilo = 0x7fffffff, ihi = -1;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
@@ -174,10 +162,10 @@ __global__ __launch_bounds__(256) void cone_fp_kernel(
     b2[k] = fmaf(a2[k], P.sig, P.tauc);
     // rays outside the detector, or parallel to the slices, contribute
     // nothing: park them in the halo
-    ok[k] = row < A.rows && col < A.cols && isfinite(a1[k]) &&
-            isfinite(a2[k]) && isfinite(b1[k]) && isfinite(b2[k]);
+    const bool ok = row < A.rows && col < A.cols && isfinite(a1[k]) &&
+                    isfinite(a2[k]) && isfinite(b1[k]) && isfinite(b2[k]);
     wscale[k] = 0.f;
-    if (!ok[k]) {
+    if (!ok) {
       a1[k] = a2[k] = 0.f;
       b1[k] = b2[k] = -1.f;
       continue;
@@ -198,33 +186,31 @@ __global__ __launch_bounds__(256) void cone_fp_kernel(
   float acc[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) acc[k] = 0.f;
-  const char* tex = reinterpret_cast<const char*>(A.tex);
+  const char* frame = A.tex + P.tex_off;
   if (ilo <= ihi) {
-    if (P.pair) {
-      if (P.colc)
-        trace<R, true, true>(P, tex, a1, b1, a2, b2, ilo, ihi, acc);
-      else
-        trace<R, false, true>(P, tex, a1, b1, a2, b2, ilo, ihi, acc);
-    } else {
-      trace<R, false, false>(P, tex, a1, b1, a2, b2, ilo, ihi, acc);
-    }
+    if (P.colc)
+      trace<R, true>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
+    else
+      trace<R, false>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
   }
 
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int32_t row = row0 + kWave * k;
     if (row >= A.rows || col >= A.cols) continue;
-    const float val = acc[k] * wscale[k];
+    float val = acc[k] * wscale[k];
     const int64_t o = ang * A.os0 + row * A.os1 + col * A.os2;
-    if (A.mode == 0) {
-      A.out[o] = val;
-    } else if (A.mode == 1) {
+    if (A.mode == 1) {
       A.out[o] += val;
-    } else {
+      continue;
+    }
+    if (A.mode == 2) {
       const float yv = A.y[ang * A.ys0 + row * A.ys1 + col * A.ys2];
       const float rv = A.R[ang * A.rs0 + row * A.rs1 + col * A.rs2];
-      A.out[o] = (val - yv) * rv;
+      val = (val - yv) * rv;
     }
+    A.out[o] = val;
+    if (A.out_dual) A.out[o + A.out_dual] = val;
   }
 }
 
@@ -233,9 +219,9 @@ __global__ __launch_bounds__(256) void cone_fp_kernel(
 namespace kk {
 
 int launch_cone_fp(const kk_plan* plan, const float* tex, float* out,
-                   const int64_t os[3], int32_t rows, int32_t cols,
-                   int32_t mode, const float* y, const int64_t ys[3],
-                   const float* R, const int64_t rs[3],
+                   const int64_t os[3], int64_t out_dual, int32_t rows,
+                   int32_t cols, int32_t mode, const float* y,
+                   const int64_t ys[3], const float* R, const int64_t rs[3],
                    int32_t rows_per_thread, hipStream_t stream) {
   if (plan == nullptr || !plan->fp_set)
     return fail(KK_ERR_NOT_SET,
@@ -247,14 +233,17 @@ int launch_cone_fp(const kk_plan* plan, const float* tex, float* out,
   if (mode == 2 && (y == nullptr || R == nullptr || ys == nullptr ||
                     rs == nullptr))
     return fail(KK_ERR_INVALID, "kk_cone_fp: residual mode needs y and R");
+  if (mode == 1 && out_dual != 0)
+    return fail(KK_ERR_INVALID, "kk_cone_fp: dual output is write-only");
   if (rows_per_thread != 1 && rows_per_thread != 2 && rows_per_thread != 4)
     rows_per_thread = rows > 128 ? 4 : (rows > 64 ? 2 : 1);
   FpArgs A{};
-  A.tex = tex;
+  A.tex = reinterpret_cast<const char*>(tex);
   A.out = out;
   A.os0 = os[0];
   A.os1 = os[1];
   A.os2 = os[2];
+  A.out_dual = out_dual;
   if (mode == 2) {
     A.y = y;
     A.ys0 = ys[0];
@@ -301,7 +290,7 @@ int launch_cone_fp(const kk_plan* plan, const float* tex, float* out,
 extern "C" int kk_cone_fp(const kk_plan* plan, const float* vol_tex,
                           float* proj, const int64_t ps[3], int32_t rows,
                           int32_t cols, int32_t additive, void* stream) {
-  return kk::launch_cone_fp(plan, vol_tex, proj, ps, rows, cols,
+  return kk::launch_cone_fp(plan, vol_tex, proj, ps, 0, rows, cols,
                             additive ? 1 : 0, nullptr, nullptr, nullptr,
                             nullptr, 0, static_cast<hipStream_t>(stream));
 }
@@ -309,8 +298,20 @@ extern "C" int kk_cone_fp(const kk_plan* plan, const float* vol_tex,
 extern "C" int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
                                    const float* y, const int64_t ys[3],
                                    const float* R, const int64_t rs[3],
-                                   float* out, const int64_t os[3],
+                                   float* res_tex, int32_t n_angles,
                                    int32_t rows, int32_t cols, void* stream) {
-  return kk::launch_cone_fp(plan, x_tex, out, os, rows, cols, 2, y, ys, R,
-                            rs, 0, static_cast<hipStream_t>(stream));
+  if (plan == nullptr || !plan->fp_set)
+    return kk::fail(KK_ERR_NOT_SET, "kk_sirt_fp_residual: no parameters");
+  if (n_angles != plan->fp_n)
+    return kk::fail(KK_ERR_SHAPE, "kk_sirt_fp_residual: angle count");
+  if (res_tex == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_sirt_fp_residual: null texture");
+  // write the residual straight into the backprojection texture: P(a, r, c)
+  // is component 0 of element (a, r+1, c+1) and component 1 of (a, r, c+1)
+  const int64_t tc = cols + kk::kHalo, tr = rows + kk::kHalo;
+  const int64_t os[3] = {2 * tr * tc, 2 * tc, 2};
+  float* out = res_tex + 2 * (kk::kHaloLo * tc + kk::kHaloLo);
+  return kk::launch_cone_fp(plan, x_tex, out, os, 1 - 2 * tc, rows, cols, 2,
+                            y, ys, R, rs, 0,
+                            static_cast<hipStream_t>(stream));
 }

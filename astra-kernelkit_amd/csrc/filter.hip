@@ -59,6 +59,7 @@ struct FilterArgs {
   int64_t is0, is1, is2;
   float* out;
   int64_t os0, os1, os2;
+  int64_t out_dual;  // if nonzero: also write out[o + out_dual]
   int64_t total_rows;
   int32_t rows, cols, n, log2n;
   const PreweightAngle* pw;  // optional fused pre-weight
@@ -140,6 +141,10 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     const float2 x = buf[c];
     out0[c * A.os2] = x.x;
     if (has1) out1[c * A.os2] = x.y;
+    if (A.out_dual) {
+      out0[c * A.os2 + A.out_dual] = x.x;
+      if (has1) out1[c * A.os2 + A.out_dual] = x.y;
+    }
   }
 }
 
@@ -271,11 +276,11 @@ extern "C" int kk_preweight(const kk_filter* f, float* proj, int32_t n,
   return KK_OK;
 }
 
-extern "C" int kk_filter_rows(const kk_filter* f, const float* in,
-                              const int64_t is[3], float* out,
-                              const int64_t os[3], int32_t n, int32_t rows,
-                              int32_t cols, int32_t fuse_preweight,
-                              void* stream) {
+static int filter_rows(const kk_filter* f, const float* in,
+                       const int64_t is[3], float* out, const int64_t os[3],
+                       int64_t out_dual, int32_t n, int32_t rows,
+                       int32_t cols, int32_t fuse_preweight,
+                       hipStream_t stream) {
   if (f == nullptr || in == nullptr || out == nullptr || is == nullptr ||
       os == nullptr)
     return kk::fail(KK_ERR_INVALID, "kk_filter_rows: null argument");
@@ -297,6 +302,7 @@ extern "C" int kk_filter_rows(const kk_filter* f, const float* in,
   A.os0 = os[0];
   A.os1 = os[1];
   A.os2 = os[2];
+  A.out_dual = out_dual;
   A.total_rows = int64_t(n) * rows;
   A.rows = rows;
   A.cols = cols;
@@ -309,8 +315,35 @@ extern "C" int kk_filter_rows(const kk_filter* f, const float* in,
     return kk::fail(KK_ERR_UNSUPPORTED, "kk_filter_rows: too many rows");
   const size_t lds = sizeof(float2) * size_t(f->n);
   hipLaunchKernelGGL(filter_rows_kernel, dim3(unsigned(pairs)), dim3(256),
-                     lds, static_cast<hipStream_t>(stream), A);
+                     lds, stream, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kk::hip_fail(e, "filter_rows_kernel");
   return KK_OK;
+}
+
+extern "C" int kk_filter_rows(const kk_filter* f, const float* in,
+                              const int64_t is[3], float* out,
+                              const int64_t os[3], int32_t n, int32_t rows,
+                              int32_t cols, int32_t fuse_preweight,
+                              void* stream) {
+  return filter_rows(f, in, is, out, os, 0, n, rows, cols, fuse_preweight,
+                     static_cast<hipStream_t>(stream));
+}
+
+extern "C" int kk_filter_rows_to_bp_texture(const kk_filter* f,
+                                            const float* in,
+                                            const int64_t is[3], float* tex,
+                                            int32_t n, int32_t rows,
+                                            int32_t cols,
+                                            int32_t fuse_preweight,
+                                            void* stream) {
+  if (tex == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_filter_rows_to_bp_texture: null");
+  // P(a, r, c) is component 0 of texture element (a, r+1, c+1) and
+  // component 1 of (a, r, c+1)
+  const int64_t tc = cols + kk::kHalo, tr = rows + kk::kHalo;
+  const int64_t os[3] = {2 * tr * tc, 2 * tc, 2};
+  float* out = tex + 2 * (kk::kHaloLo * tc + kk::kHaloLo);
+  return filter_rows(f, in, is, out, os, 1 - 2 * tc, n, rows, cols,
+                     fuse_preweight, static_cast<hipStream_t>(stream));
 }

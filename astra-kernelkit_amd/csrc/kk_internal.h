@@ -23,29 +23,49 @@ constexpr int kHalo = KK_HALO_LO + KK_HALO_HI;
 // over them, each XCD with its own L2.
 constexpr int kNumXcd = 8;
 
+// Forward-projection textures: for every ray axis a that occurs, one
+// slice-major "frame texture" T_a[i][jb][jc] of float2 over the in-plane
+// axes (b, c) = (y, z), (x, z) or (y, x) for a = x, y, z, with a zero halo
+// of KK_HALO_LO / KK_HALO_HI along b and c (none along a).  Element
+// (i, jb, jc) holds (V(i, jb-1, jc-1), V(i, jb, jc-1)): the value and its
+// b-neighbour, so the four bilinear taps of padded coordinate (jb, jc) are
+// one 16-byte load.  Frames are stored in the order x, y, z of those
+// present.
+constexpr int kFrameB[3] = {1, 0, 1};
+constexpr int kFrameC[3] = {2, 2, 0};
+
 // Per-angle forward-projection record, float32, in the ray frame (a, b, c)
-// of that angle: a = ray (major) axis, b/c = the two in-plane axes, with
-// (a, b, c) = (x, y, z), (y, x, z) or (z, y, x) for a = x, y, z; b and c are
-// swapped when that makes c the texture's contiguous axis.
+// of that angle: a = ray (major) axis, (b, c) its frame's in-plane axes.
 struct alignas(16) FpAngle {
   float qa, qb, qc, _p0;    // centre of pixel (0,0) minus source
   float ua, ub, uc, _p1;    // column step U
   float va, vb, vc, _p2;    // row step V
   float sig, taub, tauc, _p3;  // slice i <-> in-plane coordinate offsets
   float sc1, sc2, oscale, _p4; // sqrt(1 + a1^2 sc1 + a2^2 sc2) * oscale
-  int32_t na, nb, nc, pair;    // extents along a, b, c; pair: c contiguous
-  uint32_t sab, sbb, scb;      // texture strides along a, b, c in bytes
-  int32_t axis;                // world axis of a
+  int32_t na, nb, nc, axis;    // extents along a, b, c; world axis of a
+  uint64_t tex_off;            // byte offset of the frame texture T_a
+  uint32_t sab, sbb;           // slice / row byte strides of T_a
   int32_t colc;                // 1: V has no a/b part (a1, b1 row-free)
   int32_t _p5[3];
 };
 // Ray through pixel (r, c): D = q + c*U + r*V (relative to the source),
-// a1 = D_b / D_a, a2 = D_c / D_a.  Padded texture coordinate along b at
-// slice i: x_b(i) = a1 * (i + sig) + taub, with sig = 1/2 - n_a/2 - s_a and
+// a1 = D_b / D_a, a2 = D_c / D_a.  Padded coordinate along b at slice i:
+// x_b(i) = a1 * (i + sig) + taub, with sig = 1/2 - n_a/2 - s_a and
 // taub = s_b + n_b/2 - 1/2 + KK_HALO_LO (both computed in float64 from the
 // volume-centred geometry, then rounded once), so x_b = j + f reads padded
 // elements j and j + 1; 0 < x_b < n_b + 1 is the support of the volume.
 static_assert(sizeof(FpAngle) == 128, "FpAngle layout");
+
+// Padded extents of one frame texture (elements of float2).
+inline int64_t frame_elems(const int32_t shape[3], int a) {
+  return int64_t(shape[a]) * (shape[kFrameB[a]] + kHalo) *
+         (shape[kFrameC[a]] + kHalo);
+}
+
+// Backprojection texture: T[a][jv][ju] of float2, a zero halo of
+// KK_HALO_LO / KK_HALO_HI along rows and columns (none along angles);
+// element (a, jv, ju) holds (P(a, jv-1, ju-1), P(a, jv, ju-1)), so the four
+// taps at padded coordinate (jv, ju) are one 16-byte load.
 
 // Per-angle backprojection record: affine forms of the voxel centre X
 // (voxel frame, world order).  u = nu(X)/den(X) and v = nv(X)/den(X) are
@@ -69,6 +89,7 @@ struct kk_plan_s {
   kk::FpAngle* fp = nullptr;
   int32_t fp_n = 0, fp_cap = 0;
   int32_t fp_shape[3] = {0, 0, 0};
+  int32_t fp_mask = 0;  // bit a: frame texture of ray axis a is used
   bool fp_set = false;
   // backprojection table
   kk::BpAngle* bp = nullptr;
@@ -101,6 +122,11 @@ typedef float kk_float2u __attribute__((ext_vector_type(2), aligned(4)));
 // address arithmetic is spent per lane.
 __device__ __forceinline__ kk_float2u kk_ld2(const char* base, uint32_t off) {
   return *reinterpret_cast<const kk_float2u*>(base + off);
+}
+// Four consecutive floats, 8-byte aligned: one global_load_dwordx4.
+typedef float kk_float4u __attribute__((ext_vector_type(4), aligned(8)));
+__device__ __forceinline__ kk_float4u kk_ld4(const char* base, uint32_t off) {
+  return *reinterpret_cast<const kk_float4u*>(base + off);
 }
 __device__ __forceinline__ float kk_ld1(const char* base, uint32_t off) {
   return *reinterpret_cast<const float*>(base + off);

@@ -92,23 +92,13 @@ int upload_table(T** dev_ptr, int32_t* cap, const std::vector<T>& host) {
 // (nullable) gives the per-angle ray axis computed by the caller from the
 // detector centres; when null the axis is taken from |source - detector
 // centre| with the centre rebuilt from the corner, which needs rows/cols.
+// `mask_out` receives the set of frame textures the table refers to.
 static int build_fp_table(const double* params, int32_t n,
                           const int32_t* axes, int32_t rows, int32_t cols,
                           const int32_t shape[3], const double voxel_size[3],
-                          const int32_t volume_axes[3],
-                          std::vector<kk::FpAngle>& tab) {
-  if (params == nullptr || shape == nullptr || voxel_size == nullptr ||
-      volume_axes == nullptr)
+                          std::vector<kk::FpAngle>& tab, int32_t* mask_out) {
+  if (params == nullptr || shape == nullptr || voxel_size == nullptr)
     return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: null argument");
-  {
-    int seen = 0;
-    for (int k = 0; k < 3; ++k)
-      if (volume_axes[k] >= 0 && volume_axes[k] < 3)
-        seen |= 1 << volume_axes[k];
-    if (seen != 7)
-      return kk::fail(KK_ERR_INVALID,
-                      "volume_axes must be a permutation of (0, 1, 2)");
-  }
   if (n <= 0) return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: no angles");
   for (int k = 0; k < 3; ++k) {
     if (shape[k] <= 0)
@@ -116,15 +106,6 @@ static int build_fp_table(const double* params, int32_t n,
     if (!(voxel_size[k] > 0))
       return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: bad voxel size");
   }
-  // texture = padded array in array axis order; stride of each world axis
-  int64_t text[3];
-  for (int k = 0; k < 3; ++k) text[k] = shape[volume_axes[k]] + kk::kHalo;
-  if (text[0] * text[1] * text[2] >= (int64_t(1) << 31))
-    return kk::fail(KK_ERR_UNSUPPORTED,
-                    "volume too large for 32-bit texture indexing");
-  const int64_t astride[3] = {text[1] * text[2], text[2], 1};
-  int64_t tstride[3];  // element stride of each world axis
-  for (int k = 0; k < 3; ++k) tstride[volume_axes[k]] = astride[k];
   if (axes == nullptr && (rows <= 0 || cols <= 0))
     return kk::fail(KK_ERR_INVALID,
                     "kk_set_params_fp: rows/cols needed to find ray axes");
@@ -149,8 +130,40 @@ static int build_fp_table(const double* params, int32_t n,
       osc[a][2] = v[0];
     }
   }
-  static const int frame[3][3] = {{0, 1, 2}, {1, 0, 2}, {2, 1, 0}};
 
+  std::vector<int> ray_axis(n);
+  int32_t mask = 0;
+  for (int32_t k = 0; k < n; ++k) {
+    int m = 0;
+    if (axes) {
+      m = axes[k];
+      if (m < 0 || m > 2)
+        return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: bad ray axis");
+    } else {
+      double best = -1.0;
+      for (int t = 0; t < 3; ++t) {
+        const double s = params[t * n + k];
+        const double dc = params[(3 + t) * n + k] +
+                          params[(6 + t) * n + k] * (cols / 2.0) +
+                          params[(9 + t) * n + k] * (rows / 2.0);
+        const double dist = std::fabs(s - dc);
+        if (dist > best) {
+          best = dist;
+          m = t;
+        }
+      }
+    }
+    ray_axis[k] = m;
+    mask |= 1 << m;
+  }
+  // frame textures in order x, y, z of those present
+  uint64_t frame_off[3] = {0, 0, 0};
+  uint64_t off = 0;
+  for (int a = 0; a < 3; ++a) {
+    if (!(mask & (1 << a))) continue;
+    frame_off[a] = off;
+    off += uint64_t(kk::frame_elems(shape, a)) * 8;
+  }
   tab.assign(n, kk::FpAngle{});
   for (int32_t k = 0; k < n; ++k) {
     double s[3], d0[3], U[3], V[3];
@@ -160,36 +173,7 @@ static int build_fp_table(const double* params, int32_t n,
       U[t] = params[(6 + t) * n + k];
       V[t] = params[(9 + t) * n + k];
     }
-    int m;
-    if (axes) {
-      m = axes[k];
-      if (m < 0 || m > 2)
-        return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: bad ray axis");
-    } else {
-      double best = -1.0;
-      m = 0;
-      for (int t = 0; t < 3; ++t) {
-        const double dc = d0[t] + U[t] * (cols / 2.0) + V[t] * (rows / 2.0);
-        const double dist = std::fabs(s[t] - dc);
-        if (dist > best) {
-          best = dist;
-          m = t;
-        }
-      }
-    }
-    const int a = frame[m][0];
-    int b = frame[m][1], c = frame[m][2];
-    // bilinear interpolation is symmetric in (b, c): make c the contiguous
-    // axis when one of them is, so its two taps are one 8-byte load
-    double sc1 = osc[m][0], sc2 = osc[m][1];
-    if (tstride[c] != 1 && tstride[b] == 1) {
-      const int tmp = b;
-      b = c;
-      c = tmp;
-      const double ts = sc1;
-      sc1 = sc2;
-      sc2 = ts;
-    }
+    const int a = ray_axis[k], b = kk::kFrameB[a], c = kk::kFrameC[a];
     double q[3];
     for (int t = 0; t < 3; ++t) q[t] = d0[t] + 0.5 * U[t] + 0.5 * V[t] - s[t];
     kk::FpAngle& e = tab[k];
@@ -206,42 +190,39 @@ static int build_fp_table(const double* params, int32_t n,
     e.sig = float(0.5 - 0.5 * shape[a] - s[a]);
     e.taub = float(s[b] + 0.5 * shape[b] - 0.5 + kk::kHaloLo);
     e.tauc = float(s[c] + 0.5 * shape[c] - 0.5 + kk::kHaloLo);
-    e.sc1 = float(sc1);
-    e.sc2 = float(sc2);
-    e.oscale = float(osc[m][2]);
+    e.sc1 = float(osc[a][0]);
+    e.sc2 = float(osc[a][1]);
+    e.oscale = float(osc[a][2]);
     e.na = shape[a];
     e.nb = shape[b];
     e.nc = shape[c];
-    // in-slice byte offsets (j_b * sbb + j_c * scb, j <= n + 2) must fit
-    // 32 bits, and sbb/scb 24 bits (v_mul_u32_u24)
-    const int64_t sbb = 4 * tstride[b], scb = 4 * tstride[c];
-    const int64_t span = (shape[b] + 2) * sbb + (shape[c] + 2) * scb + 8;
-    if (span >= (int64_t(1) << 32) || sbb >= (int64_t(1) << 24) ||
-        scb >= (int64_t(1) << 24))
+    e.axis = a;
+    e.tex_off = frame_off[a];
+    const int64_t sbb = int64_t(shape[c] + kk::kHalo) * 8;
+    const int64_t sab = int64_t(shape[b] + kk::kHalo) * sbb;
+    // in-slice byte offsets must fit 32 bits, row strides 24 bits
+    // (v_mul_u32_u24)
+    if (sab >= (int64_t(1) << 32) || sbb >= (int64_t(1) << 24))
       return kk::fail(KK_ERR_UNSUPPORTED,
-                      "volume too large for 32-bit in-slice offsets along "
-                      "ray axis %d",
-                      m);
-    e.sab = uint32_t(4 * tstride[a]);
+                      "volume slice too large for 32-bit offsets");
+    e.sab = uint32_t(sab);
     e.sbb = uint32_t(sbb);
-    e.scb = uint32_t(scb);
-    e.pair = (tstride[c] == 1) ? 1 : 0;
-    e.axis = m;
     e.colc = (V[a] == 0.0 && V[b] == 0.0) ? 1 : 0;
   }
+  if (mask_out) *mask_out = mask;
   return KK_OK;
 }
 
 extern "C" int kk_set_params_fp(kk_plan* plan, const double* params,
                                 int32_t n, const int32_t* axes, int32_t rows,
                                 int32_t cols, const int32_t shape[3],
-                                const double voxel_size[3],
-                                const int32_t volume_axes[3]) {
+                                const double voxel_size[3]) {
   if (plan == nullptr)
     return kk::fail(KK_ERR_INVALID, "kk_set_params_fp: null plan");
   std::vector<kk::FpAngle> tab;
+  int32_t mask = 0;
   const int brc = build_fp_table(params, n, axes, rows, cols, shape,
-                                 voxel_size, volume_axes, tab);
+                                 voxel_size, tab, &mask);
   if (brc != KK_OK) return brc;
   int dev = 0;
   KK_HIP(hipGetDevice(&dev));
@@ -251,6 +232,7 @@ extern "C" int kk_set_params_fp(kk_plan* plan, const double* params,
   if (rc != KK_OK) return rc;
   plan->fp_n = n;
   for (int t = 0; t < 3; ++t) plan->fp_shape[t] = shape[t];
+  plan->fp_mask = mask;
   plan->fp_set = true;
   return KK_OK;
 }
@@ -324,19 +306,34 @@ extern "C" int kk_set_params_bp(kk_plan* plan, const double* params,
   return KK_OK;
 }
 
-extern "C" int64_t kk_texture_elems(const int64_t s[3]) {
-  if (s == nullptr || s[0] <= 0 || s[1] <= 0 || s[2] <= 0) return -1;
-  return (s[0] + kk::kHalo) * (s[1] + kk::kHalo) * (s[2] + kk::kHalo);
+extern "C" int32_t kk_fp_frame_mask(const kk_plan* plan) {
+  return (plan && plan->fp_set) ? plan->fp_mask : 0;
+}
+
+extern "C" int64_t kk_fp_texture_elems(const int32_t s[3], int32_t mask) {
+  if (s == nullptr || s[0] <= 0 || s[1] <= 0 || s[2] <= 0 || mask <= 0 ||
+      mask > 7)
+    return -1;
+  int64_t n = 0;
+  for (int a = 0; a < 3; ++a)
+    if (mask & (1 << a)) n += 2 * kk::frame_elems(s, a);
+  return n;
+}
+
+extern "C" int64_t kk_bp_texture_elems(int32_t n, int32_t rows,
+                                       int32_t cols) {
+  if (n <= 0 || rows <= 0 || cols <= 0) return -1;
+  return 2 * int64_t(n) * (rows + kk::kHalo) * (cols + kk::kHalo);
 }
 
 extern "C" int kk_fp_table(const double* params, int32_t n,
                            const int32_t* axes, int32_t rows, int32_t cols,
                            const int32_t shape[3], const double voxel_size[3],
-                           const int32_t volume_axes[3], void* out) {
+                           void* out, int32_t* mask) {
   if (out == nullptr) return kk::fail(KK_ERR_INVALID, "kk_fp_table: null");
   std::vector<kk::FpAngle> tab;
   const int rc = build_fp_table(params, n, axes, rows, cols, shape,
-                                voxel_size, volume_axes, tab);
+                                voxel_size, tab, mask);
   if (rc != KK_OK) return rc;
   std::memcpy(out, tab.data(), sizeof(kk::FpAngle) * tab.size());
   return KK_OK;

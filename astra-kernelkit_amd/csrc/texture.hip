@@ -1,33 +1,59 @@
-// Padded "texture" copies and fills.
+// Texture builders and fills.
 //
 // The reference samples CUDA texture objects with border addressing (zero
 // outside, kernelkit/kernel.py:22-32) and copies volumes/projections into
-// CUDA arrays on every set (kernel.py:91-94, projector.py:280-283, :529-542).
-// Here the equivalent is one pass that writes a C-contiguous copy of the
-// array, in the array's own axis order, with a zero halo of KK_HALO_LO
-// elements before and KK_HALO_HI after every axis.  The projector kernels
-// then clamp coordinates into the halo instead of testing bounds per tap.
+// CUDA arrays whenever they are set (kernel.py:91-94, projector.py:280-283,
+// :529-542).  Here the equivalent copies build the layouts the projector
+// kernels read (see kk_internal.h): zero-halo textures of float2 pairs
+// (value, neighbour), so that the four taps of a bilinear sample are one
+// 16-byte load and coordinates are clamped into the halo instead of being
+// bounds-tested per tap.
 #include "kk_internal.h"
 
 namespace {
 
-// One thread per texture element: x covers the fastest axis in blocks of
-// 256, grid y/z enumerate the middle and slowest axes.
-__global__ __launch_bounds__(256) void array_to_tex_kernel(
-    const float* __restrict__ src, int64_t s0, int64_t s1, int64_t s2,
-    int32_t n0, int32_t n1, int32_t n2, float* __restrict__ tex) {
-  const int32_t t2n = n2 + kk::kHalo;
-  const int32_t t1n = n1 + kk::kHalo;
-  const int32_t t2 = blockIdx.x * 256 + threadIdx.x;
-  const int32_t t1 = blockIdx.y;
-  const int32_t t0 = blockIdx.z;
-  if (t2 >= t2n) return;
-  const int32_t i0 = t0 - kk::kHaloLo, i1 = t1 - kk::kHaloLo,
-                i2 = t2 - kk::kHaloLo;
-  float v = 0.f;
-  if (i0 >= 0 && i0 < n0 && i1 >= 0 && i1 < n1 && i2 >= 0 && i2 < n2)
-    v = src[i0 * s0 + i1 * s1 + i2 * s2];
-  tex[(int64_t(t0) * t1n + t1) * t2n + t2] = v;
+// Frame texture of ray axis a: element (i, jb, jc) of a [na][nb+3][nc+3]
+// float2 array = (V(i, jb-1, jc-1), V(i, jb, jc-1)), zero outside.
+__global__ __launch_bounds__(256) void fp_frame_kernel(
+    const float* __restrict__ vol, int64_t sa, int64_t sb, int64_t sc,
+    int32_t nb, int32_t nc, float2* __restrict__ tex) {
+  const int32_t tc = nc + kk::kHalo;
+  const int32_t tb = nb + kk::kHalo;
+  const int32_t jc = blockIdx.x * 256 + threadIdx.x;
+  const int32_t jb = blockIdx.y;
+  const int32_t i = blockIdx.z;
+  if (jc >= tc) return;
+  const int32_t c = jc - kk::kHaloLo;
+  const int32_t b0 = jb - kk::kHaloLo, b1 = b0 + 1;
+  float v0 = 0.f, v1 = 0.f;
+  if (c >= 0 && c < nc) {
+    const float* p = vol + i * sa + c * sc;
+    if (b0 >= 0 && b0 < nb) v0 = p[b0 * sb];
+    if (b1 >= 0 && b1 < nb) v1 = p[b1 * sb];
+  }
+  tex[(int64_t(i) * tb + jb) * tc + jc] = make_float2(v0, v1);
+}
+
+// Projection texture: element (a, jv, ju) of a [n][rows+3][cols+3] float2
+// array = (P(a, jv-1, ju-1), P(a, jv, ju-1)), zero outside.
+__global__ __launch_bounds__(256) void bp_tex_kernel(
+    const float* __restrict__ proj, int64_t s0, int64_t s1, int64_t s2,
+    int32_t rows, int32_t cols, float2* __restrict__ tex) {
+  const int32_t tc = cols + kk::kHalo;
+  const int32_t tr = rows + kk::kHalo;
+  const int32_t ju = blockIdx.x * 256 + threadIdx.x;
+  const int32_t jv = blockIdx.y;
+  const int32_t a = blockIdx.z;
+  if (ju >= tc) return;
+  const int32_t c = ju - kk::kHaloLo;
+  const int32_t r0 = jv - kk::kHaloLo, r1 = r0 + 1;
+  float v0 = 0.f, v1 = 0.f;
+  if (c >= 0 && c < cols) {
+    const float* p = proj + a * s0 + c * s2;
+    if (r0 >= 0 && r0 < rows) v0 = p[r0 * s1];
+    if (r1 >= 0 && r1 < rows) v1 = p[r1 * s1];
+  }
+  tex[(int64_t(a) * tr + jv) * tc + ju] = make_float2(v0, v1);
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p,
@@ -40,24 +66,52 @@ __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p,
 
 }  // namespace
 
-extern "C" int kk_array_to_texture(const float* src, const int64_t sh[3],
-                                   const int64_t st[3], float* tex,
-                                   void* stream) {
-  if (src == nullptr || tex == nullptr || sh == nullptr || st == nullptr)
-    return kk::fail(KK_ERR_INVALID, "kk_array_to_texture: null argument");
-  if (sh[0] <= 0 || sh[1] <= 0 || sh[2] <= 0)
-    return kk::fail(KK_ERR_INVALID, "kk_array_to_texture: bad shape");
-  if (sh[0] + kk::kHalo > 65535 || sh[1] + kk::kHalo > 65535 ||
-      sh[2] + kk::kHalo > (int64_t(1) << 30))
-    return kk::fail(KK_ERR_UNSUPPORTED, "kk_array_to_texture: too large");
-  const dim3 grid(unsigned((sh[2] + kk::kHalo + 255) / 256),
-                  unsigned(sh[1] + kk::kHalo), unsigned(sh[0] + kk::kHalo));
-  hipLaunchKernelGGL(array_to_tex_kernel, grid, dim3(256), 0,
-                     static_cast<hipStream_t>(stream), src, st[0], st[1],
-                     st[2], int32_t(sh[0]), int32_t(sh[1]), int32_t(sh[2]),
-                     tex);
+extern "C" int kk_volume_to_fp_texture(const float* vol,
+                                       const int32_t shape[3],
+                                       const int64_t st[3], int32_t mask,
+                                       float* tex, void* stream) {
+  if (vol == nullptr || tex == nullptr || shape == nullptr || st == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_volume_to_fp_texture: null");
+  if (shape[0] <= 0 || shape[1] <= 0 || shape[2] <= 0 || mask <= 0 ||
+      mask > 7)
+    return kk::fail(KK_ERR_INVALID, "kk_volume_to_fp_texture: bad args");
+  int64_t off = 0;  // in float2 elements
+  for (int a = 0; a < 3; ++a) {
+    if (!(mask & (1 << a))) continue;
+    const int b = kk::kFrameB[a], c = kk::kFrameC[a];
+    if (shape[b] + kk::kHalo > 65535 || shape[a] > 65535)
+      return kk::fail(KK_ERR_UNSUPPORTED, "kk_volume_to_fp_texture: large");
+    const dim3 grid(unsigned((shape[c] + kk::kHalo + 255) / 256),
+                    unsigned(shape[b] + kk::kHalo), unsigned(shape[a]));
+    hipLaunchKernelGGL(fp_frame_kernel, grid, dim3(256), 0,
+                       static_cast<hipStream_t>(stream), vol, st[a], st[b],
+                       st[c], shape[b], shape[c],
+                       reinterpret_cast<float2*>(tex) + off);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return kk::hip_fail(e, "fp_frame_kernel");
+    off += kk::frame_elems(shape, a);
+  }
+  return KK_OK;
+}
+
+extern "C" int kk_projections_to_bp_texture(const float* proj, int32_t n,
+                                            int32_t rows, int32_t cols,
+                                            const int64_t st[3], float* tex,
+                                            void* stream) {
+  if (proj == nullptr || tex == nullptr || st == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_projections_to_bp_texture: null");
+  if (n <= 0 || rows <= 0 || cols <= 0)
+    return kk::fail(KK_ERR_INVALID, "kk_projections_to_bp_texture: shape");
+  if (rows + kk::kHalo > 65535 || n > 65535)
+    return kk::fail(KK_ERR_UNSUPPORTED,
+                    "kk_projections_to_bp_texture: too large");
+  const dim3 grid(unsigned((cols + kk::kHalo + 255) / 256),
+                  unsigned(rows + kk::kHalo), unsigned(n));
+  hipLaunchKernelGGL(bp_tex_kernel, grid, dim3(256), 0,
+                     static_cast<hipStream_t>(stream), proj, st[0], st[1],
+                     st[2], rows, cols, reinterpret_cast<float2*>(tex));
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return kk::hip_fail(e, "array_to_tex_kernel");
+  if (e != hipSuccess) return kk::hip_fail(e, "bp_tex_kernel");
   return KK_OK;
 }
 

@@ -53,32 +53,39 @@ _SIGNATURES = {
     "kk_plan_destroy": (ctypes.c_int, [_vp]),
     "kk_set_params_fp": (ctypes.c_int, [_vp, _c_f64p, ctypes.c_int32, _c_i32p,
                                         ctypes.c_int32, ctypes.c_int32,
-                                        _c_i32p, _c_f64p, _c_i32p]),
+                                        _c_i32p, _c_f64p]),
+    "kk_fp_frame_mask": (ctypes.c_int32, [_vp]),
     "kk_set_params_bp": (ctypes.c_int, [_vp, _c_f64p, ctypes.c_int32]),
-    "kk_texture_elems": (ctypes.c_int64, [_c_i64p]),
-    "kk_array_to_texture": (ctypes.c_int, [_vp, _c_i64p, _c_i64p, _vp, _vp]),
+    "kk_fp_texture_elems": (ctypes.c_int64, [_c_i32p, ctypes.c_int32]),
+    "kk_bp_texture_elems": (ctypes.c_int64, [ctypes.c_int32, ctypes.c_int32,
+                                             ctypes.c_int32]),
+    "kk_volume_to_fp_texture": (ctypes.c_int, [_vp, _c_i32p, _c_i64p,
+                                               ctypes.c_int32, _vp, _vp]),
+    "kk_projections_to_bp_texture": (ctypes.c_int, [_vp, ctypes.c_int32,
+                                                    ctypes.c_int32,
+                                                    ctypes.c_int32, _c_i64p,
+                                                    _vp, _vp]),
     "kk_fill": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_float, _vp]),
     "kk_cone_fp": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, ctypes.c_int32,
                                   ctypes.c_int32, ctypes.c_int32, _vp]),
-    "kk_cone_bp": (ctypes.c_int, [_vp, _vp, _c_i32p, ctypes.c_int32,
-                                  ctypes.c_int32, ctypes.c_int32, _vp,
-                                  _c_i32p, _c_i64p, ctypes.c_float,
-                                  ctypes.c_int32, _vp]),
+    "kk_cone_bp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
+                                  ctypes.c_int32, _vp, _c_i32p, _c_i64p,
+                                  ctypes.c_float, ctypes.c_int32, _vp]),
     "kk_sirt_fp_residual": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, _vp,
-                                           _c_i64p, _vp, _c_i64p,
+                                           _c_i64p, _vp, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_int32,
                                            _vp]),
-    "kk_sirt_bp_update": (ctypes.c_int, [_vp, _vp, _c_i32p, ctypes.c_int32,
+    "kk_sirt_bp_update": (ctypes.c_int, [_vp, _vp, ctypes.c_int32,
                                          ctypes.c_int32, ctypes.c_int32, _vp,
                                          _c_i32p, _c_i64p, _vp, _c_i64p, _vp,
                                          _c_i64p, ctypes.c_float,
                                          ctypes.c_float, ctypes.c_int32,
-                                         ctypes.c_int32, _vp, _c_i64p,
+                                         ctypes.c_int32, _vp, ctypes.c_int32,
                                          ctypes.c_float, _vp]),
     "kk_recip_or_zero": (ctypes.c_int, [_vp, ctypes.c_int64, _vp]),
     "kk_fp_table": (ctypes.c_int, [_c_f64p, ctypes.c_int32, _c_i32p,
                                    ctypes.c_int32, ctypes.c_int32, _c_i32p,
-                                   _c_f64p, _c_i32p, _vp]),
+                                   _c_f64p, _vp, _c_i32p]),
     "kk_bp_table": (ctypes.c_int, [_c_f64p, ctypes.c_int32, _vp]),
     "kk_filter_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
     "kk_filter_destroy": (ctypes.c_int, [_vp]),
@@ -90,6 +97,11 @@ _SIGNATURES = {
     "kk_filter_rows": (ctypes.c_int, [_vp, _vp, _c_i64p, _vp, _c_i64p,
                                       ctypes.c_int32, ctypes.c_int32,
                                       ctypes.c_int32, ctypes.c_int32, _vp]),
+    "kk_filter_rows_to_bp_texture": (ctypes.c_int, [_vp, _vp, _c_i64p, _vp,
+                                                    ctypes.c_int32,
+                                                    ctypes.c_int32,
+                                                    ctypes.c_int32,
+                                                    ctypes.c_int32, _vp]),
 }
 
 
@@ -228,8 +240,8 @@ FP_ANGLE_DTYPE = np.dtype([
     ("va", "<f4"), ("vb", "<f4"), ("vc", "<f4"), ("_p2", "<f4"),
     ("sig", "<f4"), ("taub", "<f4"), ("tauc", "<f4"), ("_p3", "<f4"),
     ("sc1", "<f4"), ("sc2", "<f4"), ("oscale", "<f4"), ("_p4", "<f4"),
-    ("na", "<i4"), ("nb", "<i4"), ("nc", "<i4"), ("pair", "<i4"),
-    ("sab", "<u4"), ("sbb", "<u4"), ("scb", "<u4"), ("axis", "<i4"),
+    ("na", "<i4"), ("nb", "<i4"), ("nc", "<i4"), ("axis", "<i4"),
+    ("tex_off", "<u8"), ("sab", "<u4"), ("sbb", "<u4"),
     ("colc", "<i4"), ("_p5", "<i4", (3,))])
 BP_ANGLE_DTYPE = np.dtype([
     ("nux", "<f4"), ("nuy", "<f4"), ("nuz", "<f4"), ("nuw", "<f4"),
@@ -239,19 +251,21 @@ BP_ANGLE_DTYPE = np.dtype([
 assert FP_ANGLE_DTYPE.itemsize == 128 and BP_ANGLE_DTYPE.itemsize == 64
 
 
-def fp_table(params12: np.ndarray, ray_axes, rows, cols, shape, voxel_size,
-             volume_axes) -> np.ndarray:
-    """Host copy of the forward-projection device table."""
+def fp_table(params12: np.ndarray, ray_axes, rows, cols, shape,
+             voxel_size):
+    """Host copy of the forward-projection device table and its frame
+    mask."""
     params12 = np.ascontiguousarray(params12, dtype=np.float64)
     n = params12.shape[1]
     out = np.zeros(n, dtype=FP_ANGLE_DTYPE)
     axes = (None if ray_axes is None
             else np.ascontiguousarray(ray_axes, dtype=np.int32))
+    mask = ctypes.c_int32(0)
     check(lib().kk_fp_table(f64_ptr(params12), n,
                             None if axes is None else i32_ptr(axes), rows,
                             cols, i32(shape), f64(voxel_size),
-                            i32(volume_axes), out.ctypes.data_as(_vp)))
-    return out
+                            out.ctypes.data_as(_vp), ctypes.byref(mask)))
+    return out, int(mask.value)
 
 
 def bp_table(params: np.ndarray) -> np.ndarray:

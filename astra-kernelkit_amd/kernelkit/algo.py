@@ -20,9 +20,9 @@ import numpy as np
 from kernelkit import _native
 from kernelkit.geom.proj import Beam, ProjectionGeometry
 from kernelkit.geom.vol import VolumeGeometry
-from kernelkit.kernel import (BaseKernel, Texture, as_device_array,
+from kernelkit.kernel import (BaseKernel, ProjectionTexture, as_device_array,
                               default_device, is_device_array)
-from kernelkit.kernels.cone_bp import VoxelDrivenConeBP, _world_strides
+from kernelkit.kernels.cone_bp import VoxelDrivenConeBP
 from kernelkit.operator import XrayTransform
 from kernelkit.processing import filter_into
 from kernelkit.projector import BackProjector, ForwardProjector
@@ -86,11 +86,9 @@ def bp(projections: Any, projection_geometry, volume_geometry: VolumeGeometry,
     if filter is not None:
         if projection_axes != (0, 1, 2):
             raise NotImplementedError()
-        tex = Texture(proj.shape, device=proj.device, layered=True)
-        _native.check(_native.lib().kk_fill(
-            _native._vp(tex.ptr), tex.data.numel(), 0.0,
-            _native.stream_ptr()))
-        filter_into(proj, tex.interior(), geoms=list(projection_geometry),
+        tex = ProjectionTexture(*proj.shape, device=proj.device)
+        tex.zero()
+        filter_into(proj, tex, geoms=list(projection_geometry),
                     filter=filter)
         ptor._textures = tex
     else:
@@ -179,18 +177,14 @@ class _SirtState:
         self.fptor.volume = self.x
         self.xtex = self.fptor._texture
         self.rtex = self.bptor._textures
-        _native.check(lib.kk_fill(_native._vp(self.rtex.ptr),
-                                  self.rtex.data.numel(), 0.0, st))
-        self.res = self.rtex.interior()
+        self.rtex.zero()
+        self.res = self.rtex.values()
         self.mask = None if mask is None else as_device_array(mask)
         self.lo = 0.0 if min_constraint is None else float(min_constraint)
         self.hi = 0.0 if max_constraint is None else float(max_constraint)
         self.clip_lo = 0 if min_constraint is None else 1
         self.clip_hi = 0 if max_constraint is None else 1
         self.scale = float(volume_geometry.voxel_volume())
-        xi = self.xtex.interior()
-        self.xtex_origin = xi.data_ptr()
-        self.xtex_strides = _world_strides(xi, (0, 1, 2))
 
     def iterate(self):
         lib = _native.lib()
@@ -200,20 +194,18 @@ class _SirtState:
             fk._plan.ptr, _native._vp(self.xtex.ptr),
             _native._vp(self.y.data_ptr()), _native.i64(self.y.stride()),
             _native._vp(self.R.data_ptr()), _native.i64(self.R.stride()),
-            _native._vp(self.res.data_ptr()), _native.i64(self.res.stride()),
-            self.rows, self.cols, st))
+            _native._vp(self.rtex.ptr), self.n, self.rows, self.cols, st))
         mask_ptr = None if self.mask is None else _native._vp(
             self.mask.data_ptr())
         mask_st = None if self.mask is None else _native.i64(
             self.mask.stride())
         _native.check(lib.kk_sirt_bp_update(
-            bk._plan.ptr, _native._vp(self.rtex.ptr), _native.i32((0, 1, 2)),
+            bk._plan.ptr, _native._vp(self.rtex.ptr),
             self.n, self.rows, self.cols, _native._vp(self.x.data_ptr()),
             _native.i32(self.shape), _native.i64(self.x.stride()),
             _native._vp(self.C.data_ptr()), _native.i64(self.C.stride()),
             mask_ptr, mask_st, self.lo, self.hi, self.clip_lo, self.clip_hi,
-            _native._vp(self.xtex_origin), _native.i64(self.xtex_strides),
-            self.scale, st))
+            _native._vp(self.xtex.ptr), self.xtex.mask, self.scale, st))
 
 
 def sirt(projections: Any, projection_geometry: Any,

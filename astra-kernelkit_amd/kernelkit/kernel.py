@@ -7,9 +7,11 @@ through CUDA texture objects with border addressing (``:22-32``, ``:47-119``).
 
 Here kernels are ahead-of-time gfx950 code in ``libkernelkit_hip.so``:
 ``compile()`` loads the library and creates a plan that owns the per-angle
-device tables; a "texture" is a zero-halo padded copy of the array in its own
-axis order (:class:`Texture`), which the HIP kernels sample with explicit
-float32 interpolation and no per-tap bounds tests.
+device tables.  The HIP kernels sample gather layouts built from the arrays
+(:class:`VolumeTexture`, :class:`ProjectionTexture`: zero-halo float2 pairs,
+so one 16-byte load fetches the four bilinear taps) with explicit float32
+interpolation and no per-tap bounds tests; :class:`Texture` is the plain
+snapshot that ``copy_to_texture`` returns.
 """
 from __future__ import annotations
 
@@ -63,72 +65,144 @@ def as_device_array(x, dtype=None, copy: bool = False):
 
 
 class Texture:
-    """Zero-halo padded float32 copy of a 3-D device array.
-
-    The copy keeps the array's axis order (like the CUDA array behind a
-    texture object) and pads every axis with ``HALO_LO`` zeros before and
-    ``HALO_HI`` after, so kernels can clamp coordinates into the halo instead
-    of testing bounds.  ``shape`` is the unpadded array shape.
+    """Snapshot of an array, the analogue of a CUDA array behind a texture
+    object (``kernel.py:47-119``): a contiguous float32 device copy in the
+    array's own axis order.  Kernels convert it to their gather layout
+    (:class:`VolumeTexture` / :class:`ProjectionTexture`) on use.
     """
 
-    def __init__(self, shape, device=None, layered: bool = False):
+    def __init__(self, array, layered: bool = False):
         torch = _torch()
-        self.shape = tuple(int(s) for s in shape)
-        if len(self.shape) != 3 or min(self.shape) <= 0:
-            raise ValueError(f"texture shape must be 3-D, got {self.shape}")
+        dev = as_device_array(array)
+        if dev.dtype != torch.float32:
+            raise ValueError(
+                f"Array must be of type float32, got {dev.dtype} instead.")
+        if dev.dim() == 2:
+            dev = dev.unsqueeze(0)
+        self.array = dev.contiguous().clone()
+        self.shape = tuple(self.array.shape)
         self.layered = bool(layered)
-        n = _native.lib().kk_texture_elems(_native.i64(self.shape))
+        self.version = 0
+
+    def copy_from(self, array) -> None:
+        """Refresh the snapshot with new data of the same shape."""
+        dev = as_device_array(array)
+        if dev.dim() == 2:
+            dev = dev.unsqueeze(0)
+        if tuple(dev.shape) != self.shape:
+            raise ValueError(f"array shape {tuple(dev.shape)} does not "
+                             f"match texture shape {self.shape}")
+        self.array.copy_(dev)
+        self.version += 1
+
+
+class VolumeTexture:
+    """Forward-projection layout of a volume: one slice-major frame of
+    (value, neighbour) float2 pairs with a zero halo per ray axis in
+    ``mask`` (bit a = world axis a); see ``include/kernelkit_hip.h``."""
+
+    def __init__(self, world_shape, mask: int, device=None):
+        torch = _torch()
+        self.world_shape = tuple(int(s) for s in world_shape)
+        self.mask = int(mask)
+        n = _native.lib().kk_fp_texture_elems(_native.i32(self.world_shape),
+                                              self.mask)
         if n < 0:
-            raise ValueError(f"bad texture shape {self.shape}")
+            raise ValueError(f"bad volume shape {self.world_shape} or frame "
+                             f"mask {mask}")
         self.data = torch.empty(int(n), dtype=torch.float32,
                                 device=device or default_device())
-
-    @property
-    def padded_shape(self):
-        return tuple(s + _native.HALO for s in self.shape)
 
     @property
     def ptr(self):
         return self.data.data_ptr()
 
-    def interior(self):
-        """View of the unpadded region (shape ``self.shape``)."""
-        lo = _native.HALO_LO
-        full = self.data.view(self.padded_shape)
-        return full[lo:lo + self.shape[0], lo:lo + self.shape[1],
-                    lo:lo + self.shape[2]]
+    def covers(self, mask: int) -> bool:
+        return (int(mask) & ~self.mask) == 0
 
-    def copy_from(self, array, stream=None) -> None:
-        """Asynchronously copy ``array`` (same shape, any strides) in."""
+    def update(self, volume, volume_axes=(0, 1, 2), stream=None) -> None:
+        """Copy ``volume`` (array axis k = world axis ``volume_axes[k]``)."""
         torch = _torch()
-        if not (isinstance(array, torch.Tensor) and array.is_cuda):
-            array = as_device_array(array)
-        if array.dtype != torch.float32:
+        if not (isinstance(volume, torch.Tensor) and volume.is_cuda):
+            volume = as_device_array(volume)
+        if volume.dtype != torch.float32:
             raise ValueError(
-                f"Array must be of type float32, got {array.dtype} instead.")
-        if array.dim() == 2:
-            array = array.unsqueeze(0)
-        if tuple(array.shape) != self.shape:
-            raise ValueError(f"array shape {tuple(array.shape)} does not "
-                             f"match texture shape {self.shape}")
-        _native.check(_native.lib().kk_array_to_texture(
-            _native._vp(array.data_ptr()), _native.i64(array.shape),
-            _native.i64(array.stride()), _native._vp(self.ptr),
+                f"Volume must be of type float32, got {volume.dtype}.")
+        wshape = [0, 0, 0]
+        wstride = [0, 0, 0]
+        for k, ax in enumerate(volume_axes):
+            wshape[ax] = volume.shape[k]
+            wstride[ax] = volume.stride(k)
+        if tuple(wshape) != self.world_shape:
+            raise ValueError(f"volume of world shape {tuple(wshape)} does "
+                             f"not match texture shape {self.world_shape}")
+        _native.check(_native.lib().kk_volume_to_fp_texture(
+            _native._vp(volume.data_ptr()), _native.i32(self.world_shape),
+            _native.i64(wstride), self.mask, _native._vp(self.ptr),
             _native.stream_ptr(stream)))
-        self._source = array  # the caller's array stays referenced
 
-    def copy_to(self, array, stream=None) -> None:
-        """Copy the interior back into ``array`` (same shape)."""
-        array.copy_(self.interior().reshape(array.shape))
+
+class ProjectionTexture:
+    """Backprojection layout of projections: ``[angle][rows+3][cols+3]`` of
+    (value, next-row) float2 pairs with a zero halo; see
+    ``include/kernelkit_hip.h``."""
+
+    def __init__(self, n_angles: int, rows: int, cols: int, device=None):
+        torch = _torch()
+        self.shape = (int(n_angles), int(rows), int(cols))
+        n = _native.lib().kk_bp_texture_elems(*self.shape)
+        if n < 0:
+            raise ValueError(f"bad projection shape {self.shape}")
+        self.data = torch.empty(int(n), dtype=torch.float32,
+                                device=device or default_device())
+
+    @property
+    def ptr(self):
+        return self.data.data_ptr()
+
+    def zero(self, stream=None) -> None:
+        _native.check(_native.lib().kk_fill(
+            _native._vp(self.ptr), self.data.numel(), 0.0,
+            _native.stream_ptr(stream)))
+
+    def update(self, projections, projection_axes=(0, 1, 2),
+               stream=None) -> None:
+        """Copy projections (array axis k = logical axis
+        ``projection_axes[k]`` of (angle, row, col))."""
+        torch = _torch()
+        if not (isinstance(projections, torch.Tensor) and projections.is_cuda):
+            projections = as_device_array(projections)
+        if projections.dtype != torch.float32:
+            raise ValueError("Projections must be of type float32, got "
+                             f"{projections.dtype}.")
+        lshape = [0, 0, 0]
+        lstride = [0, 0, 0]
+        for k, ax in enumerate(projection_axes):
+            lshape[ax] = projections.shape[k]
+            lstride[ax] = projections.stride(k)
+        if tuple(lshape) != self.shape:
+            raise ValueError(f"projections of shape {tuple(lshape)} do not "
+                             f"match texture shape {self.shape}")
+        _native.check(_native.lib().kk_projections_to_bp_texture(
+            _native._vp(projections.data_ptr()), *self.shape,
+            _native.i64(lstride), _native._vp(self.ptr),
+            _native.stream_ptr(stream)))
+
+    def values(self):
+        """The stored projections as an (angle, row, col) strided view."""
+        n, rows, cols = self.shape
+        h = _native.HALO
+        full = self.data.view(n, rows + h, cols + h, 2)
+        lo = _native.HALO_LO
+        return full[:, lo:lo + rows, lo:lo + cols, 0]
 
 
 def copy_to_texture(array, texture_type: str = "array",
                     layered: bool = False) -> Texture:
-    """Padded texture copy of a float32 array (``kernel.py:47-119``).
+    """Texture snapshot of a float32 array (``kernel.py:47-119``).
 
     ``texture_type`` 'array' expects a 3-D array, 'array2d'/'pitch2d' a 2-D
-    one (stored as a single layer); pitched or strided views are accepted
-    as they are (strides are honoured by the copy).
+    one (stored as a single layer); pitched or strided views are accepted.
     """
     torch = _torch()
     if not isinstance(array, (np.ndarray, torch.Tensor)):
@@ -155,11 +229,7 @@ def copy_to_texture(array, texture_type: str = "array",
             raise ValueError("layered textures must be 3-D")
     else:
         raise ValueError(f"Type `{texture_type}` not understood.")
-    dev = as_device_array(array)
-    shape = tuple(dev.shape) if dev.dim() == 3 else (1, *dev.shape)
-    tex = Texture(shape, device=dev.device, layered=layered)
-    tex.copy_from(dev)
-    return tex
+    return Texture(array, layered=layered)
 
 
 @dataclass(frozen=True, order=True)

@@ -17,8 +17,8 @@ from kernelkit.geom import normalize_
 from kernelkit.geom.proj import ProjectionGeometrySequence
 from kernelkit.geom.vol import VolumeGeometry
 from kernelkit.kernel import (BaseKernel, KernelMemoryUnpreparedException,
-                              KernelNotCompiledException, Texture,
-                              is_device_array)
+                              KernelNotCompiledException, ProjectionTexture,
+                              Texture, is_device_array)
 
 
 def _world_strides(tensor, volume_axes) -> list:
@@ -138,10 +138,11 @@ class VoxelDrivenConeBP(BaseKernel):
                  additive: bool = True, stream=None):
         """Backproject the projection texture into ``volume`` (in place).
 
-        ``textures`` is a :class:`~kernelkit.kernel.Texture` of the
-        projection array (in ``projection_axes`` order).  Like the reference
-        kernel the result is accumulated (``cuda/cone_bp.cu:156-158``)
-        unless ``additive=False``.
+        ``textures`` is a :class:`~kernelkit.kernel.ProjectionTexture`, or a
+        plain :class:`~kernelkit.kernel.Texture` snapshot of the projection
+        array (in ``projection_axes`` order), which is converted on use.
+        Like the reference kernel the result is accumulated
+        (``cuda/cone_bp.cu:156-158``) unless ``additive=False``.
         """
         import torch
         if isinstance(volume, torch.Tensor) and volume.is_cuda:
@@ -158,17 +159,16 @@ class VoxelDrivenConeBP(BaseKernel):
         if not self.is_compiled:
             raise KernelNotCompiledException(
                 "Please compile the kernel with `compile()`.")
-        if not isinstance(textures, Texture):
+        if not isinstance(textures, (Texture, ProjectionTexture)):
             raise ValueError(
-                "Please give in a kernelkit Texture of the projections, given "
+                "Please give in a kernelkit texture of the projections, given "
                 f"is a '{type(textures).__name__}'.")
         if not self._params_are_set:
             raise KernelMemoryUnpreparedException(
                 "Please set the parameters with `set_params()`.")
-        logical = [0, 0, 0]
-        for k, ax in enumerate(self._proj_axs):
-            logical[ax] = textures.shape[k]
-        n, rows, cols = logical
+        if isinstance(textures, Texture):
+            textures = self._from_snapshot(textures)
+        n, rows, cols = textures.shape
         expect = tuple(int(volume_geometry.shape[i]) for i in self._vol_axs)
         if tuple(volume.shape) != expect:
             raise ValueError(
@@ -176,12 +176,34 @@ class VoxelDrivenConeBP(BaseKernel):
                 f"geometry {tuple(volume_geometry.shape)} with axes "
                 f"{self._vol_axs}.")
         _native.check(_native.lib().kk_cone_bp(
-            self._plan.ptr, _native._vp(textures.ptr),
-            _native.i32(self._proj_axs), n, rows, cols,
+            self._plan.ptr, _native._vp(textures.ptr), n, rows, cols,
             _native._vp(volume.data_ptr()), _native.i32(volume_geometry.shape),
             _native.i64(_world_strides(volume, self._vol_axs)),
             float(volume_geometry.voxel_volume()), 1 if additive else 0,
             _native.stream_ptr(stream)))
+
+    def projection_texture(self, projections,
+                           texture: ProjectionTexture | None = None
+                           ) -> ProjectionTexture:
+        """Build (or refresh) the backprojection texture of projections
+        given in ``projection_axes`` order."""
+        logical = [0, 0, 0]
+        for k, ax in enumerate(self._proj_axs):
+            logical[ax] = projections.shape[k]
+        if texture is None or texture.shape != tuple(logical):
+            dev = projections.device if is_device_array(projections) else None
+            texture = ProjectionTexture(*logical, device=dev)
+        texture.update(projections, self._proj_axs)
+        return texture
+
+    def _from_snapshot(self, snap: Texture) -> ProjectionTexture:
+        key = (id(snap), snap.version)
+        cached = getattr(self, "_snap_cache", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        tex = self.projection_texture(snap.array)
+        self._snap_cache = (key, tex)
+        return tex
 
     @staticmethod
     def geoms2params(projection_geometry, volume_geometry: VolumeGeometry,

@@ -17,7 +17,7 @@ from kernelkit.geom.proj import ProjectionGeometrySequence
 from kernelkit.geom.vol import VolumeGeometry
 from kernelkit.kernel import (BaseKernel, KernelMemoryUnpreparedException,
                               KernelNotCompiledException, Texture,
-                              is_device_array)
+                              VolumeTexture, is_device_array)
 
 
 class FPParams(tuple):
@@ -190,11 +190,32 @@ class RayDrivenConeFP(BaseKernel):
                     else _native.i32_ptr(self._ray_axes))
         _native.check(_native.lib().kk_set_params_fp(
             self._plan.ptr, _native.f64_ptr(self._params), n, axes_ptr,
-            rows, cols, _native.i32(vg.shape), _native.f64(vg.voxel_size),
-            _native.i32(self._vol_axs)))
+            rows, cols, _native.i32(vg.shape), _native.f64(vg.voxel_size)))
         self._uploaded_key = key
 
-    def __call__(self, volume_texture: Texture, projection_geometry,
+    @property
+    def frame_mask(self) -> int:
+        """Frames of the volume texture the uploaded parameters use (bit a:
+        ray axis a), 0 before the first upload."""
+        if not self.is_compiled or self._uploaded_key is None:
+            return 0
+        return int(_native.lib().kk_fp_frame_mask(self._plan.ptr))
+
+    def volume_texture(self, volume, volume_geometry: VolumeGeometry,
+                       texture: VolumeTexture | None = None) -> VolumeTexture:
+        """Build (or refresh) the forward-projection texture of ``volume``
+        (in ``volume_axes`` order) for the uploaded parameters; before the
+        first upload the frames of the x and y ray axes are built."""
+        mask = self.frame_mask or 0b011
+        shape = tuple(int(s) for s in volume_geometry.shape)
+        if (texture is None or texture.world_shape != shape
+                or not texture.covers(mask)):
+            dev = volume.device if is_device_array(volume) else None
+            texture = VolumeTexture(shape, mask, device=dev)
+        texture.update(volume, self._vol_axs)
+        return texture
+
+    def __call__(self, volume_texture, projection_geometry,
                  volume_geometry: VolumeGeometry, projections,
                  projections_pitch: tuple | None = None, *,
                  additive: bool = True, stream=None):
@@ -215,22 +236,37 @@ class RayDrivenConeFP(BaseKernel):
         if not self._params_are_set:
             raise KernelMemoryUnpreparedException(
                 "Please set the projection geometry with `set_params()`.")
-        if not isinstance(volume_texture, Texture):
-            raise TypeError("`volume_texture` must be a kernelkit Texture.")
+        if not isinstance(volume_texture, (Texture, VolumeTexture)):
+            raise TypeError("`volume_texture` must be a kernelkit texture.")
         n = self._params.shape[1]
         if len(projection_geometry) != n:
             raise ValueError(
                 f"{len(projection_geometry)} projection geometries but "
                 f"parameters for {n} angles.")
         rows, cols = detector_shape(projection_geometry)
-        expect_vol = tuple(int(volume_geometry.shape[i])
-                           for i in self._vol_axs)
-        if tuple(volume_texture.shape) != expect_vol:
+        world = tuple(int(s) for s in volume_geometry.shape)
+        if isinstance(volume_texture, Texture):
+            expect_vol = tuple(world[i] for i in self._vol_axs)
+            if tuple(volume_texture.shape) != expect_vol:
+                raise ValueError(
+                    f"Volume texture shape {volume_texture.shape} does not "
+                    f"match volume geometry {world} with axes "
+                    f"{self._vol_axs}.")
+        elif volume_texture.world_shape != world:
             raise ValueError(
-                f"Volume texture shape {volume_texture.shape} does not match "
-                f"volume geometry {tuple(volume_geometry.shape)} with axes "
-                f"{self._vol_axs}.")
+                f"Volume texture shape {volume_texture.world_shape} does not"
+                f" match volume geometry {world}.")
         self._upload(rows, cols, volume_geometry)
+        if isinstance(volume_texture, Texture):
+            # a plain snapshot: convert it to this kernel's gather layout
+            volume_texture = self._from_snapshot(volume_texture,
+                                                 volume_geometry)
+        elif not volume_texture.covers(self.frame_mask):
+            raise ValueError(
+                "The volume texture lacks frames for ray axes of this "
+                f"geometry (has {volume_texture.mask:#b}, needs "
+                f"{self.frame_mask:#b}); rebuild it after setting the "
+                "geometry.")
         logical = (n, rows, cols)
         expect = tuple(logical[i] for i in self._proj_axs)
         if isinstance(projections, (list, tuple)):
@@ -274,7 +310,17 @@ class RayDrivenConeFP(BaseKernel):
         self._launch(volume_texture, projections, rows, cols, additive,
                      stream)
 
-    def _launch(self, tex: Texture, proj, rows, cols, additive, stream):
+    def _from_snapshot(self, snap: Texture, vg) -> VolumeTexture:
+        key = (id(snap), snap.version, self.frame_mask)
+        cached = getattr(self, "_snap_cache", None)
+        if cached is not None and cached[0] == key:
+            return cached[1]
+        tex = self.volume_texture(snap.array, vg)
+        self._snap_cache = (key, tex)
+        return tex
+
+    def _launch(self, tex: VolumeTexture, proj, rows, cols, additive,
+                stream):
         strides = _logical_strides(proj, self._proj_axs)
         _native.check(_native.lib().kk_cone_fp(
             self._plan.ptr, _native._vp(tex.ptr), _native._vp(proj.data_ptr()),

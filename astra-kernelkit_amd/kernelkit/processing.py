@@ -150,15 +150,20 @@ def filter(projections, verbose: bool = False,
         writeback()
 
 
-def filter_into(projections, out, geoms=None, filter: str = "ramlak",
+def filter_into(projections, texture, geoms=None, filter: str = "ramlak",
                 detector_piercings=None) -> None:
-    """Fused FDK pre-processing: ``out = filter(preweight(projections))``.
+    """Fused FDK pre-processing into a backprojection texture:
+    ``texture <- filter(preweight(projections))``.
 
-    ``projections`` and ``out`` are ``(angle, row, col)`` device tensors
-    (views allowed, e.g. the interior of a backprojection texture).  The
-    pre-weight is applied on load when ``geoms`` is given.
+    ``projections`` is an ``(angle, row, col)`` device tensor (views
+    allowed); ``texture`` a :class:`~kernelkit.kernel.ProjectionTexture` of
+    the same shape whose halo is zero.  The pre-weight is applied on load
+    when ``geoms`` is given.
     """
     n, rows, cols = projections.shape
+    if tuple(texture.shape) != (n, rows, cols):
+        raise ValueError(f"texture of shape {texture.shape} does not match "
+                         f"projections of shape {(n, rows, cols)}")
     num = padded_length(cols)
     # a private table set: the pre-weights are specific to this call
     f = _native.Filter()
@@ -169,8 +174,7 @@ def filter_into(projections, out, geoms=None, filter: str = "ramlak",
         P = np.ascontiguousarray(preweight_params(geoms, detector_piercings))
         _native.check(_native.lib().kk_filter_set_preweight(
             f.ptr, _native.f64_ptr(P), n))
-    _native.check(_native.lib().kk_filter_rows(
+    _native.check(_native.lib().kk_filter_rows_to_bp_texture(
         f.ptr, _native._vp(projections.data_ptr()),
-        _native.i64(projections.stride()), _native._vp(out.data_ptr()),
-        _native.i64(out.stride()), n, rows, cols,
-        1 if geoms is not None else 0, _native.stream_ptr()))
+        _native.i64(projections.stride()), _native._vp(texture.ptr), n, rows,
+        cols, 1 if geoms is not None else 0, _native.stream_ptr()))

@@ -19,7 +19,7 @@ import numpy as np
 
 from kernelkit.geom.proj import ProjectionGeometry, ProjectionGeometrySequence
 from kernelkit.geom.vol import VolumeGeometry
-from kernelkit.kernel import (BaseKernel, Texture, as_device_array,
+from kernelkit.kernel import (BaseKernel, VolumeTexture, as_device_array,
                               is_device_array)
 from kernelkit.kernels.cone_bp import VoxelDrivenConeBP
 from kernelkit.kernels.cone_fp import RayDrivenConeFP
@@ -215,11 +215,20 @@ class ForwardProjector(Projector):
         if not is_device_array(value):
             raise TypeError("Volume must be a device tensor.")
         self._vol_ref = value
+        self._build_texture()
+
+    def _build_texture(self):
+        value = self._vol_ref
+        world = [0, 0, 0]
+        for k, ax in enumerate(self.volume_axes):
+            world[ax] = int(value.shape[k])
+        mask = self._K.frame_mask or 0b011
         tex = getattr(self, "_texture", None)
-        if tex is None or tex.shape != tuple(value.shape):
-            tex = Texture(value.shape, device=value.device)
+        if (tex is None or tex.world_shape != tuple(world)
+                or not tex.covers(mask)):
+            tex = VolumeTexture(world, mask, device=value.device)
             self._texture = tex
-        tex.copy_from(value)
+        tex.update(value, self.volume_axes)
 
     @volume.deleter
     def volume(self):
@@ -248,8 +257,18 @@ class ForwardProjector(Projector):
 
     def __call__(self, additive: bool = False):
         """Project the volume into the projections (overwrite unless
-        ``additive``)."""
+        ``additive``).
+
+        If the geometry was set after the volume and needs ray-axis frames
+        the texture lacks, the texture is rebuilt from the volume first.
+        """
         self._require_geometry()
+        if not hasattr(self, "_texture"):
+            raise VolumeNotSetError(
+                "Volume not set. Use the `volume` setter before calling the "
+                "projector.")
+        if not self._texture.covers(self._K.frame_mask):
+            self._build_texture()
         self._K(self._texture, self._proj_geom_seq, self._vol_geom,
                 self.projections, additive=additive)
 
@@ -314,10 +333,11 @@ class BackProjector(Projector):
 
     @property
     def projections(self):
-        """Interior view of the projection texture (``projector.py:427-446``:
-        the reference returns its CUDA array(s))."""
+        """The stored projections as an (angle, row, col) view of the
+        texture (``projector.py:427-446``: the reference returns its CUDA
+        array(s))."""
         try:
-            return self._textures.interior()
+            return self._textures.values()
         except AttributeError:
             raise ProjectionsNotSetError("No projections set in the "
                                          "projector.")
@@ -356,12 +376,8 @@ class BackProjector(Projector):
             raise ValueError("Can only use layered texture when the first "
                              "projection axis is 0.")
         value = as_device_array(value)
-        tex = getattr(self, "_textures", None)
-        if tex is None or tex.shape != tuple(value.shape):
-            tex = Texture(value.shape, device=value.device,
-                          layered=self._texture_type == "layered")
-            self._textures = tex
-        tex.copy_from(value)
+        self._textures = self._K.projection_texture(
+            value, getattr(self, "_textures", None))
 
     @projections.deleter
     def projections(self):

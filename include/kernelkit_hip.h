@@ -33,9 +33,16 @@ extern "C" {
 #define KK_ERR_SHAPE (-4)       /* shape mismatch (ValueError)               */
 #define KK_ERR_UNSUPPORTED (-5) /* unsupported case (NotImplementedError)    */
 
-/* Padded "texture" layouts: every axis gets KK_HALO_LO zero elements before
- * and KK_HALO_HI after the data (replaces the CUDA texture's border mode,
- * kernelkit/kernel.py:22-32). */
+/* Textures.  The reference samples CUDA texture objects with border
+ * addressing (zero outside, kernelkit/kernel.py:22-32).  Here a "texture" is
+ * a library-defined zero-halo layout of float2 pairs (value, neighbour),
+ * KK_HALO_LO elements before and KK_HALO_HI after along the interpolated
+ * axes, built from any strided array by the kk_*_to_*_texture calls:
+ *  - FP texture: for each ray axis a present in the geometry (bit a of the
+ *    frame mask), a slice-major frame [n_a][n_b+3][n_c+3] of float2 with
+ *    (b, c) = (y, z), (x, z), (y, x) for a = x, y, z; frames in x, y, z order;
+ *  - BP texture: [angle][rows+3][cols+3] of float2 (value, next row).
+ * Sizes are given in floats by kk_fp_texture_elems / kk_bp_texture_elems. */
 #define KK_HALO_LO 1
 #define KK_HALO_HI 2
 
@@ -59,53 +66,60 @@ int kk_plan_destroy(kk_plan* plan);
  * units.  ray_axes (nullable): per-angle ray axis 0/1/2, the reference's
  * argmax |source - detector centre| (kernels/cone_fp.py:264-285); when NULL
  * it is computed here from the corner plus rows/cols.  vol_shape and
- * voxel_size are world (x, y, z); volume_axes[k] is the world axis of array
- * axis k of the volume (kernels/cone_fp.py:43-47).  Together they fix the
- * texture strides and the output scale (kernels/cone_fp.py:287-300). */
+ * voxel_size are world (x, y, z); they fix the texture layout and the
+ * output scale (kernels/cone_fp.py:287-300). */
 int kk_set_params_fp(kk_plan* plan, const double* params, int32_t n_angles,
                      const int32_t* ray_axes, int32_t rows, int32_t cols,
-                     const int32_t vol_shape[3], const double voxel_size[3],
-                     const int32_t volume_axes[3]);
+                     const int32_t vol_shape[3], const double voxel_size[3]);
+
+/* Frame mask (bit a: ray axis a occurs) of the plan's FP parameters. */
+int32_t kk_fp_frame_mask(const kk_plan* plan);
 
 /* VoxelDrivenConeBP.set_params (kernelkit/kernels/cone_bp.py:168-190).
  * params: n_angles x 12 float64 as returned by VoxelDrivenConeBP.geoms2params
  * (kernels/cone_bp.py:401-421): per angle numU.xyzw, numV.xyzw, den.xyzw. */
 int kk_set_params_bp(kk_plan* plan, const double* params, int32_t n_angles);
 
-/* Number of floats of the padded texture of a 3-D array of `shape`. */
-int64_t kk_texture_elems(const int64_t shape[3]);
+/* Texture sizes in floats (-1 on bad arguments). */
+int64_t kk_fp_texture_elems(const int32_t vol_shape[3], int32_t frame_mask);
+int64_t kk_bp_texture_elems(int32_t n_angles, int32_t rows, int32_t cols);
 
-/* Array -> padded texture (the CUDA-array copy of copy_to_texture,
- * kernelkit/kernel.py:47-119).  The texture keeps the array's own axis
- * order, C-contiguous, with a zero halo of KK_HALO_LO/KK_HALO_HI on every
- * axis; `strides` are the source's element strides (views, pitched rows and
- * permutations are fine).  The whole texture is written, halo included. */
-int kk_array_to_texture(const float* src, const int64_t shape[3],
-                        const int64_t strides[3], float* tex, void* stream);
+/* Volume -> FP texture (the volume setter's copy_to_texture,
+ * kernelkit/projector.py:266-283).  vol_strides: element strides of world
+ * x, y, z in the source (any volume_axes permutation or pitch).  Writes
+ * every element of the frames in frame_mask, halo included. */
+int kk_volume_to_fp_texture(const float* vol, const int32_t vol_shape[3],
+                            const int64_t vol_strides[3], int32_t frame_mask,
+                            float* tex, void* stream);
+
+/* Projections -> BP texture (the projections setter, projector.py:503-556).
+ * strides: element strides of (angle, row, col) in the source (any
+ * projection_axes permutation or pitch). */
+int kk_projections_to_bp_texture(const float* proj, int32_t n_angles,
+                                 int32_t rows, int32_t cols,
+                                 const int64_t strides[3], float* tex,
+                                 void* stream);
 
 /* value-fill of n floats (a kernel, not hipMemsetAsync: graph-safe). */
 int kk_fill(float* ptr, int64_t n, float value, void* stream);
 
 /* Ray-driven Joseph forward projection, all angles in one launch
  * (cone_fp<DIR_X|Y|Z>, kernelkit/cuda/cone_fp.cu:73-207, launched by
- * kernels/cone_fp.py:195-262).  vol_tex: texture of the volume array whose
- * shape/axes were given to kk_set_params_fp.  proj: output addressed by
- * (angle, row, col) element strides (any projection_axes or pitch).
- * additive != 0: proj += result; else proj = result (the reference
- * zero-fills first, projector.py:337-343). */
+ * kernels/cone_fp.py:195-262).  vol_tex: FP texture holding at least the
+ * plan's frames.  proj: output addressed by (angle, row, col) element strides
+ * (any projection_axes or pitch).  additive != 0: proj += result; else
+ * proj = result (the reference zero-fills first, projector.py:337-343). */
 int kk_cone_fp(const kk_plan* plan, const float* vol_tex, float* proj,
                const int64_t proj_strides[3], int32_t rows, int32_t cols,
                int32_t additive, void* stream);
 
 /* Voxel-driven backprojection, all angles in one launch (cone_bp,
  * kernelkit/cuda/cone_bp.cu:35-159, kernels/cone_bp.py:292-302).
- * proj_tex: texture of the projection array, whose array axis k is
- * (angle, row, col)[projection_axes[k]] (cone_bp.cu:112-138); n_angles must
- * equal the parameter count.  vol: output addressed by world (x, y, z)
- * element strides; output_scale: the voxel volume (kernels/cone_bp.py:294).
- * additive != 0: vol += result; else vol = result (projector.py:586-587). */
-int kk_cone_bp(const kk_plan* plan, const float* proj_tex,
-               const int32_t projection_axes[3], int32_t n_angles,
+ * proj_tex: BP texture; n_angles must equal the parameter count.  vol:
+ * output addressed by world (x, y, z) element strides; output_scale: the
+ * voxel volume (kernels/cone_bp.py:294).  additive != 0: vol += result;
+ * else vol = result (projector.py:586-587). */
+int kk_cone_bp(const kk_plan* plan, const float* proj_tex, int32_t n_angles,
                int32_t rows, int32_t cols, float* vol,
                const int32_t vol_shape[3], const int64_t vol_strides[3],
                float output_scale, int32_t additive, void* stream);
@@ -115,39 +129,39 @@ int kk_cone_bp(const kk_plan* plan, const float* proj_tex,
 /* The per-angle device records that kk_set_params_fp / kk_set_params_bp
  * would upload, written to `out` (n_angles x 128 bytes for the FP table,
  * n_angles x 64 bytes for the BP table; layouts FpAngle / BpAngle of
- * csrc/kk_internal.h).  Used by CPU tests to check the table arithmetic. */
+ * csrc/kk_internal.h); frame_mask (nullable) receives the FP frame mask.
+ * Used by CPU tests to check the table arithmetic. */
 int kk_fp_table(const double* params, int32_t n_angles,
                 const int32_t* ray_axes, int32_t rows, int32_t cols,
                 const int32_t vol_shape[3], const double voxel_size[3],
-                const int32_t volume_axes[3], void* out);
+                void* out, int32_t* frame_mask);
 int kk_bp_table(const double* params, int32_t n_angles, void* out);
 
 /* ---- SIRT (kernelkit/algo.py:226-368), fused ------------------------- */
 
-/* out = (FP(x) - y) * R for every ray: the residual step of one SIRT
- * iteration (algo.py:339-344) fused into the forward projection epilogue.
- * y, R, out: (angle, row, col) element strides; out may be the interior of
- * a projection texture. */
+/* res = (FP(x) - y) * R for every ray: the residual step of one SIRT
+ * iteration (algo.py:339-344) fused into the forward projection epilogue,
+ * written straight into the BP texture res_tex (whose halo must be zero).
+ * y, R: (angle, row, col) element strides. */
 int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
                         const float* y, const int64_t y_strides[3],
                         const float* R, const int64_t R_strides[3],
-                        float* out, const int64_t out_strides[3],
-                        int32_t rows, int32_t cols, void* stream);
+                        float* res_tex, int32_t n_angles, int32_t rows,
+                        int32_t cols, void* stream);
 
 /* x = clip(mask * (x - C * BP(res))) (algo.py:356-362), fused into the
  * backprojection epilogue.  mask (nullable), C, x: world strides; clip_lo /
  * clip_hi select the min/max constraints.  x_tex (nullable): the FP texture
- * of x at voxel (0, 0, 0) with world strides x_tex_strides, refreshed with
- * the new x so the next forward projection needs no copy. */
+ * of x with frames x_tex_mask, refreshed with the new x so the next forward
+ * projection needs no copy. */
 int kk_sirt_bp_update(const kk_plan* plan, const float* res_tex,
-                      const int32_t projection_axes[3], int32_t n_angles,
-                      int32_t rows, int32_t cols, float* x,
+                      int32_t n_angles, int32_t rows, int32_t cols, float* x,
                       const int32_t vol_shape[3], const int64_t x_strides[3],
                       const float* C, const int64_t C_strides[3],
                       const float* mask, const int64_t mask_strides[3],
                       float lo, float hi, int32_t clip_lo, int32_t clip_hi,
-                      float* x_tex, const int64_t x_tex_strides[3],
-                      float output_scale, void* stream);
+                      float* x_tex, int32_t x_tex_mask, float output_scale,
+                      void* stream);
 
 /* p = 1/p with +inf mapped to 0, n contiguous floats (algo.py:324-332). */
 int kk_recip_or_zero(float* ptr, int64_t n, void* stream);
@@ -183,6 +197,15 @@ int kk_filter_rows(const kk_filter* filter, const float* in,
                    const int64_t out_strides[3], int32_t n_angles,
                    int32_t rows, int32_t cols, int32_t fuse_preweight,
                    void* stream);
+
+/* Same, writing the filtered projections straight into a BP texture (FDK:
+ * algo.py:94-126 without the intermediate copy); the texture's halo must
+ * be zero. */
+int kk_filter_rows_to_bp_texture(const kk_filter* filter, const float* in,
+                                 const int64_t in_strides[3], float* tex,
+                                 int32_t n_angles, int32_t rows,
+                                 int32_t cols, int32_t fuse_preweight,
+                                 void* stream);
 
 #ifdef __cplusplus
 }

@@ -185,3 +185,49 @@ def test_thin_volume_and_single_pixel(device):
     assert_parity(_host(A(_dev(vol))), ref, label="fp/single-pixel")
     ref_bp = oracle.cone_bp(ref, pgs, vg)
     assert_parity(_host(A.T(_dev(ref))), ref_bp, label="bp/single-pixel")
+
+
+def test_z_major_rays(device):
+    """Rays along z (a top-down orbit) use the z frame texture."""
+    g0 = kk.rotate(workloads.circular_geometry(
+        1, sod=120.0, odd=60.0, rows=40, cols=36, pixel=1.2)[0],
+        pitch=np.pi / 2)
+    pgs = [kk.rotate(g0, yaw=t) for t in np.linspace(0, 2 * np.pi, 6, False)]
+    vg = kk.resolve_volume_geometry(shape=(24, 28, 20), voxel_size=1.0)
+    vol = workloads.ellipsoid_phantom(vg.shape, n=4, seed=9)
+    A = kk.XrayTransform(pgs, vg)
+    ref = oracle.cone_fp(vol, pgs, vg)
+    assert_parity(_host(A(_dev(vol))), ref, label="fp/z-major")
+    ref_bp = oracle.cone_bp(ref, pgs, vg)
+    assert_parity(_host(A.T(_dev(ref))), ref_bp, label="bp/z-major")
+
+
+def test_mixed_ray_axes(device):
+    """Angles of all three ray axes in one operator (three frames)."""
+    base = workloads.circular_geometry(8, sod=120.0, odd=60.0, rows=40,
+                                       cols=36, pixel=1.2)
+    pgs = base + [kk.rotate(g, pitch=np.pi / 2) for g in base[:3]]
+    vg = kk.resolve_volume_geometry(shape=(24, 28, 20), voxel_size=1.0)
+    vol = workloads.ellipsoid_phantom(vg.shape, n=4, seed=10)
+    A = kk.XrayTransform(pgs, vg)
+    assert A.projector.kernel.frame_mask == 0b111
+    ref = oracle.cone_fp(vol, pgs, vg)
+    assert_parity(_host(A(_dev(vol))), ref, label="fp/mixed")
+
+
+def test_volume_set_before_geometry(device):
+    """The texture is rebuilt when the geometry needs frames it lacks."""
+    g0 = kk.rotate(workloads.circular_geometry(
+        1, sod=120.0, odd=60.0, rows=40, cols=36, pixel=1.2)[0],
+        pitch=np.pi / 2)
+    vg = kk.resolve_volume_geometry(shape=(24, 28, 20), voxel_size=1.0)
+    vol = workloads.ellipsoid_phantom(vg.shape, n=4, seed=11)
+    fp = kk.ForwardProjector()
+    fp.volume = _dev(vol)            # frames default to x and y
+    fp.volume_geometry = vg
+    fp.projection_geometry = [g0]    # needs the z frame
+    out = _dev(np.zeros((1, 40, 36)))
+    fp.projections = out
+    fp()
+    assert_parity(_host(out), oracle.cone_fp(vol, [g0], vg),
+                  label="fp/late-geometry")

@@ -6,8 +6,6 @@ This pins the host-side table derivation (ray axes, b/c swap, slice
 offsets, texel-centre shifts, fast-path detection) and the kernels' index
 arithmetic (halo clamping, slice-range clipping) without a GPU.
 """
-import itertools
-
 import numpy as np
 import pytest
 
@@ -26,17 +24,25 @@ def _texture(arr):
     return np.pad(np.asarray(arr, f32), [(lo, hi)] * 3)
 
 
-def emulate_fp(vol_array, table, rows, cols):
+_FRAMES = {0: (0, 1, 2), 1: (1, 0, 2), 2: (2, 1, 0)}  # (a, b, c) per axis
+
+
+def emulate_fp(vol_world, table, rows, cols):
     """cone_fp_kernel in float32 numpy: returns proj[angle, row, col].
 
-    Coordinates are padded-texture coordinates (halo folded into the
-    intercepts), clamped to [0, n + 1]; slice i is padded slice i + 1."""
-    tex = _texture(vol_array).ravel()
+    Reads the frame texture of each angle's ray axis, i.e. the volume in
+    (a, b, c) order padded by the halo along b and c; coordinates are padded
+    coordinates (halo folded into the intercepts) clamped to [0, n + 1]."""
+    lo, hi = _native.HALO_LO, _native.HALO_HI
+    frames = {a: np.pad(np.transpose(np.asarray(vol_world, f32), abc),
+                        [(0, 0), (lo, hi), (lo, hi)])
+              for a, abc in _FRAMES.items()}
     n = len(table)
     out = np.zeros((n, rows, cols), f32)
     r = np.arange(rows, dtype=f32)[:, None]
     c = np.arange(cols, dtype=f32)[None, :]
     for k, P in enumerate(table):
+        T = frames[int(P["axis"])]
         Da = r * f32(P["va"]) + (c * f32(P["ua"]) + f32(P["qa"]))
         Db = r * f32(P["vb"]) + (c * f32(P["ub"]) + f32(P["qb"]))
         Dc = r * f32(P["vc"]) + (c * f32(P["uc"]) + f32(P["qc"]))
@@ -46,24 +52,21 @@ def emulate_fp(vol_array, table, rows, cols):
         b1 = (a1 * f32(P["sig"]) + f32(P["taub"])).astype(f32)
         b2 = (a2 * f32(P["sig"]) + f32(P["tauc"])).astype(f32)
         na, nb, nc = int(P["na"]), int(P["nb"]), int(P["nc"])
-        sab, sbb, scb = int(P["sab"]), int(P["sbb"]), int(P["scb"])
-        assert sab % 4 == 0 and sbb % 4 == 0 and scb % 4 == 0
-        sa, sb, sc = sab // 4, sbb // 4, scb // 4
+        assert T.shape == (na, nb + lo + hi, nc + lo + hi)
+        assert int(P["sbb"]) == 8 * (nc + lo + hi)
+        assert int(P["sab"]) == int(P["sbb"]) * (nb + lo + hi)
         acc = np.zeros((rows, cols), f32)
         for i in range(na):  # the kernel's range clipping only skips zeros
             xb = np.clip(a1 * f32(i) + b1, 0, nb + 1).astype(f32)
             xc = np.clip(a2 * f32(i) + b2, 0, nc + 1).astype(f32)
-            jb = np.floor(xb)
-            jc = np.floor(xc)
+            jb = np.floor(xb).astype(np.int64)
+            jc = np.floor(xc).astype(np.int64)
             fb = xb - jb
             fc = xc - jc
-            idx = (i + _native.HALO_LO) * sa + jb.astype(np.int64) * sb + \
-                jc.astype(np.int64) * sc
-            p00, p01 = tex[idx], tex[idx + sc]
-            p10, p11 = tex[idx + sb], tex[idx + sb + sc]
-            v0 = fc * (p01 - p00) + p00
-            v1 = fc * (p11 - p10) + p10
-            acc += fb * (v1 - v0) + v0
+            Ti = T[i]
+            l0 = fb * (Ti[jb + 1, jc] - Ti[jb, jc]) + Ti[jb, jc]
+            l1 = fb * (Ti[jb + 1, jc + 1] - Ti[jb, jc + 1]) + Ti[jb, jc + 1]
+            acc += fc * (l1 - l0) + l0
         scale = np.sqrt(a1 * a1 * f32(P["sc1"]) + a2 * a2 * f32(P["sc2"])
                         + f32(1)) * f32(P["oscale"])
         out[k] = acc * scale
@@ -110,11 +113,13 @@ def _case(n_angles=12, shape=(28, 32, 24), rows=36, cols=40, **pert):
     return pgs, vg, vol
 
 
-def _fp_table(pgs, vg, volume_axes=(0, 1, 2)):
+def _fp_table(pgs, vg):
     P = kk.kernels.RayDrivenConeFP.geoms2params(pgs, vg)
     det = pgs[0].detector
-    return _native.fp_table(np.stack(P), P.ray_axes, det.rows, det.cols,
-                            vg.shape, vg.voxel_size, volume_axes)
+    table, mask = _native.fp_table(np.stack(P), P.ray_axes, det.rows,
+                                   det.cols, vg.shape, vg.voxel_size)
+    assert mask == int(np.bitwise_or.reduce(1 << table["axis"]))
+    return table
 
 
 @pytest.mark.parametrize("pert", [{}, {"tilt_deg": 2.0, "drift_px": 4.0},
@@ -125,12 +130,18 @@ def test_fp_emulation_matches_oracle(pert):
     assert_parity(got, oracle.cone_fp(vol, pgs, vg), label=f"fp-emu {pert}")
 
 
-@pytest.mark.parametrize("axes", list(itertools.permutations((0, 1, 2))))
-def test_fp_emulation_volume_axes(axes):
-    pgs, vg, vol = _case(n_angles=6, shape=(16, 20, 12), rows=24, cols=28)
-    got = emulate_fp(np.transpose(vol, axes), _fp_table(pgs, vg, axes), 24,
-                     28)
-    assert_parity(got, oracle.cone_fp(vol, pgs, vg), label=f"axes {axes}")
+def test_fp_emulation_z_major():
+    """A geometry whose rays run along z uses the z frame (b, c) = (y, x)."""
+    g0 = kk.rotate(workloads.circular_geometry(
+        1, sod=120.0, odd=60.0, rows=24, cols=28, pixel=1.2)[0],
+        pitch=np.pi / 2)
+    pgs = [kk.rotate(g0, yaw=t) for t in np.linspace(0, 2 * np.pi, 4, False)]
+    vg = kk.resolve_volume_geometry(shape=(16, 20, 12), voxel_size=1.0)
+    vol = workloads.ellipsoid_phantom(vg.shape, n=3, seed=1)
+    table = _fp_table(pgs, vg)
+    assert set(table["axis"].tolist()) == {2}
+    got = emulate_fp(vol, table, 24, 28)
+    assert_parity(got, oracle.cone_fp(vol, pgs, vg), label="z-major")
 
 
 @pytest.mark.parametrize("pert", [{}, {"tilt_deg": 2.0, "drift_px": 4.0}])
@@ -152,9 +163,7 @@ def test_full_config_fp_table_precision():
     t = _fp_table(sub, vg)
     fr = oracle.normalized_frames(sub, vg)
     for k, P in enumerate(t):
-        a, b, c = {0: (0, 1, 2), 1: (1, 0, 2), 2: (2, 1, 0)}[int(P["axis"])]
-        if int(P["sbb"]) == 4:
-            b, c = c, b
+        a, b, c = _FRAMES[int(P["axis"])]
         s, d0 = fr["s"][k], fr["d0"][k]
         U, V = fr["U"][k], fr["V"][k]
         for (r, col) in ((0, 0), (383, 400), (767, 767)):
