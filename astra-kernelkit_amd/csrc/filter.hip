@@ -196,11 +196,18 @@ extern "C" int kk_filter_set_response(kk_filter* f, const double* H,
     const double ang = -2.0 * pi * double(k) / double(n);
     tw[k] = make_float2(float(std::cos(ang)), float(std::sin(ang)));
   }
+  // Only the even part He[k] = (H[k] + H[-k]) / 2 is kept.  For a real row
+  // p, Re(IDFT(DFT(p) H)) = IDFT(DFT(p) He): the odd part of a real H only
+  // adds an imaginary component, which the real part drops.  Windows such
+  // as fftshift(hanning(N)) are not even (hanning is symmetric about
+  // (N-1)/2), so this matters; with an even response the kernel may pack
+  // two real rows into one complex transform.
   std::vector<float> hb(n);
   for (int32_t k = 0; k < n; ++k) {
     int32_t rev = 0;
     for (int b = 0; b < log2n; ++b) rev |= ((k >> b) & 1) << (log2n - 1 - b);
-    hb[k] = float(H[rev] / double(n));
+    const double he = 0.5 * (H[rev] + H[(n - rev) & (n - 1)]);
+    hb[k] = float(he / double(n));
   }
   if (f->tw) hipFree(f->tw);
   if (f->hb) hipFree(f->hb);

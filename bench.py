@@ -1,0 +1,280 @@
+"""Cone-beam FP/BP throughput on one MI355X: 512^3 volume, 720 angles,
+768x768 detector (SOD 1000, ODD 500, pixel 1.0), seeded 10-ellipsoid
+phantom, FP32.
+
+One step = one forward projection of the volume into all 720 projections
+plus one voxel-driven backprojection of those projections into the volume,
+each through the projector API with resident device inputs: the texture
+conversion of the input (volume -> FP frame textures, projections -> BP
+texture) and the kernel.  ``value`` is GUPS over the step:
+``2 * 512^3 * 720 / t_step`` (an update is one (voxel, angle) pair; the step
+does two passes of updates).
+
+The JSON line also carries:
+
+* ``roofline`` for the dominant kernel (the forward projection): achieved
+  gather bytes per second = 16 B per bilinear sample x samples per launch /
+  average kernel time (HIP events on the launching stream), against the
+  aggregate L2 read rate 8 XCD x 16 channels x 128 B/clk x 2.4 GHz;
+  ``traffic`` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE +
+  WRITE_SIZE counters (scaled by a plain-copy calibration), read from
+  ``profiles/cone_fp_bp_512cube_720_traffic.json`` when present;
+* ``cpu_baseline``: the FP64 NumPy oracle on one host core, timed on a
+  bounded sample of the same workload (one angle, a band of detector rows
+  for FP and a slab of the volume for BP), in GUPS.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "astra-kernelkit_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = ("cone FP/BP GUPS at 512³/720 proj; batched + autograd vs "
+          "plain, same GPU")
+L2_PEAK_GBS = 8 * 16 * 128 * 2.4  # GB/s: 8 XCD x 16 ch x 128 B/clk x 2.4 GHz
+HBM_PEAK_GBS = 8000.0
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "cone_fp_bp_512cube_720_traffic.json")
+
+
+def _log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def fp_samples(pgs, vg) -> int:
+    """Bilinear samples of one forward projection: every ray samples each
+    slice along its angle's major axis."""
+    import numpy as np
+    from kernelkit.geom.proj import ProjectionGeometrySequence
+    import oracle
+    frames = oracle.normalized_frames(pgs, vg)
+    axes = oracle.fp_major_axes(frames)
+    seq = ProjectionGeometrySequence.fromList(list(pgs))
+    rays = int(seq.detector.rows[0]) * int(seq.detector.cols[0])
+    shape = np.asarray(vg.shape, dtype=np.int64)
+    return int(rays * shape[axes].sum())
+
+
+def cpu_baseline(pgs, vg, vol):
+    """FP64 oracle on one core over a bounded sample of the workload."""
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    import kernelkit as kk
+    import oracle
+    from kernelkit.geom.proj import Detector, ProjectionGeometry
+
+    g = pgs[0]
+    rows, cols = int(g.detector.rows), int(g.detector.cols)
+    band = 96
+    r0 = rows // 2 - band // 2
+    shift = (r0 + band / 2.0 - rows / 2.0) * float(g.detector.pixel_height)
+    g_band = ProjectionGeometry(
+        g.source_position, np.asarray(g.detector_position) + shift *
+        np.asarray(g.v), Detector(band, cols, g.detector.pixel_height,
+                                  g.detector.pixel_width), u=g.u, v=g.v)
+    nx = int(vg.shape[0])
+    slab = nx // 4
+    vs = np.asarray(vg.voxel_size, dtype=np.float64)
+    emin = np.asarray(vg.extent_min, dtype=np.float64)
+    emax = np.asarray(vg.extent_max, dtype=np.float64)
+    x0 = (nx - slab) // 2
+    smin = emin.copy()
+    smax = emax.copy()
+    smin[0] = emin[0] + x0 * vs[0]
+    smax[0] = smin[0] + slab * vs[0]
+    vg_slab = kk.VolumeGeometry((slab,) + tuple(vg.shape[1:]), vs, smin, smax)
+    with threadpool_limits(limits=1):
+        t0 = time.perf_counter()
+        y = oracle.cone_fp(vol.astype(np.float64), [g_band], vg)
+        t_fp = time.perf_counter() - t0
+        proj = np.zeros((1, rows, cols))
+        proj[0, r0:r0 + band] = y[0]
+        t0 = time.perf_counter()
+        oracle.cone_bp(proj, [g], vg_slab)
+        t_bp = time.perf_counter() - t0
+    n_vox = int(np.prod(vg.shape))
+    # per-angle time of the whole workload, by exact linearity in rays and
+    # in voxels
+    t_fp_angle = t_fp * rows / band
+    t_bp_angle = t_bp * nx / slab
+    gups = 2 * n_vox / (t_fp_angle + t_bp_angle) / 1e9
+    return {
+        "value": round(gups, 6), "unit": "GUPS", "cores": 1, "kind": "port",
+        "sample": (f"FP64 NumPy oracle, 1 angle: FP over {band} of {rows} "
+                   f"detector rows ({t_fp:.1f} s), BP over a {slab}-slice "
+                   f"x-slab of the {nx}^3 volume ({t_bp:.1f} s); scaled to "
+                   f"one full angle of FP+BP by ray/voxel count"),
+    }
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--config", default="full",
+                    help="workload: full (default), small, diff")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    metavar="MIB",
+                    help="after the timed steps, run one plain device copy "
+                         "of MIB MiB (the reference kernel for scaling the "
+                         "FETCH_SIZE/WRITE_SIZE counters)")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the cone FP/BP path is built and "
+              "timed on a single MI355X (--gpus 1).", file=sys.stderr)
+        return 2
+
+    import numpy as np
+    import torch
+
+    import kernelkit as kk
+    from kernelkit import workloads
+
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        return 1
+    dev = torch.device("cuda", 0)
+    pgs, vg = workloads.config_geometry(args.config)
+    n, rows, cols = len(pgs), int(pgs[0].detector.rows), \
+        int(pgs[0].detector.cols)
+    shape = tuple(int(s) for s in vg.shape)
+    _log(f"workload {args.config}: volume {shape}, {n} angles, "
+         f"detector {rows}x{cols}")
+    vol = workloads.ellipsoid_phantom(shape, n=10, seed=0)
+    x = torch.from_numpy(vol).to(dev)
+    y = torch.empty((n, rows, cols), dtype=torch.float32, device=dev)
+    v = torch.empty(shape, dtype=torch.float32, device=dev)
+
+    fptor = kk.ForwardProjector()
+    fptor.projection_geometry = pgs
+    fptor.volume_geometry = vg
+    fptor.projections = y
+    bptor = kk.BackProjector()
+    bptor.projection_geometry = pgs
+    bptor.volume_geometry = vg
+    bptor.volume = v
+
+    stream = torch.cuda.current_stream(dev)
+
+    def ev():
+        e = torch.cuda.Event(enable_timing=True)
+        e.record(stream)
+        return e
+
+    def step():
+        e0 = ev()
+        fptor.volume = x          # volume -> FP frame textures
+        e1 = ev()
+        fptor()                   # forward projection kernel
+        e2 = ev()
+        bptor.projections = y     # projections -> BP texture
+        e3 = ev()
+        bptor()                   # backprojection kernel
+        e4 = ev()
+        return e0, e1, e2, e3, e4
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    _log(f"warmup {args.warmup} done")
+
+    events = []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        events.append(step())
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    _log(f"timed {args.steps} steps in {wall:.3f} s")
+    if args.copy_calibration > 0:
+        n_el = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n_el, dtype=torch.float32, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)   # one read + one write kernel
+        torch.cuda.synchronize()
+        _log(f"copy calibration: {args.copy_calibration} MiB")
+        del src, dst
+
+    def mean(i, j):
+        return float(np.mean([e[i].elapsed_time(e[j]) for e in events]))
+
+    t_ftex, t_fp, t_btex, t_bp = mean(0, 1), mean(1, 2), mean(2, 3), \
+        mean(3, 4)
+    ms_step = wall * 1e3 / args.steps
+    updates = int(np.prod(shape)) * n
+    value = 2 * updates / (ms_step * 1e-3) / 1e9
+    samples = fp_samples(pgs, vg)
+    fp_bytes = 16 * samples
+    bp_bytes = 16 * updates
+    achieved = fp_bytes / (t_fp * 1e-3) / 1e9
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE) and args.config == "full":
+        with open(TRAFFIC_FILE) as f:
+            traffic = json.load(f).get("cone_fp_kernel_bytes")
+    compulsory_fp = 4 * (int(np.prod(shape)) + n * rows * cols)
+    out = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 10-ellipsoid phantom, circular orbit",
+        "config": {
+            "workload": (f"cone FP + voxel-driven BP, volume "
+                         f"{shape[0]}x{shape[1]}x{shape[2]}, {n} angles, "
+                         f"detector {rows}x{cols}"),
+            "name": args.config,
+            "fp_gups": round(updates / (t_fp * 1e-3) / 1e9, 3),
+            "bp_gups": round(updates / (t_bp * 1e-3) / 1e9, 3),
+            "fp_kernel_ms": round(t_fp, 3),
+            "bp_kernel_ms": round(t_bp, 3),
+            "fp_texture_ms": round(t_ftex, 3),
+            "bp_texture_ms": round(t_btex, 3),
+        },
+        "roofline": {
+            "bound": "l2",
+            "kernel": "cone_fp_kernel",
+            "achieved": round(achieved, 1),
+            "peak": round(L2_PEAK_GBS, 1),
+            "unit": "GB/s",
+            "frac": round(achieved / L2_PEAK_GBS, 4),
+            "traffic": traffic,
+            "algorithmic_bytes": fp_bytes,
+            "bytes_per_update": round(fp_bytes / updates, 4),
+            "hbm_compulsory_GBs": round(
+                compulsory_fp / (t_fp * 1e-3) / 1e9, 1),
+            "hbm_peak": HBM_PEAK_GBS,
+            "bp_kernel": {
+                "achieved": round(bp_bytes / (t_bp * 1e-3) / 1e9, 1),
+                "frac": round(bp_bytes / (t_bp * 1e-3) / 1e9 / L2_PEAK_GBS,
+                              4),
+                "bytes_per_update": 16,
+            },
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        _log("cpu baseline (FP64 oracle, 1 core) ...")
+        out["cpu_baseline"] = cpu_baseline(pgs, vg, vol)
+    print(json.dumps(out), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
