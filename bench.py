@@ -73,7 +73,7 @@ def cpu_baseline(pgs, vg, vol):
 
     g = pgs[0]
     rows, cols = int(g.detector.rows), int(g.detector.cols)
-    band = 96
+    band = rows // 2
     r0 = rows // 2 - band // 2
     shift = (r0 + band / 2.0 - rows / 2.0) * float(g.detector.pixel_height)
     g_band = ProjectionGeometry(
@@ -81,7 +81,7 @@ def cpu_baseline(pgs, vg, vol):
         np.asarray(g.v), Detector(band, cols, g.detector.pixel_height,
                                   g.detector.pixel_width), u=g.u, v=g.v)
     nx = int(vg.shape[0])
-    slab = nx // 4
+    slab = nx // 2
     vs = np.asarray(vg.voxel_size, dtype=np.float64)
     emin = np.asarray(vg.extent_min, dtype=np.float64)
     emax = np.asarray(vg.extent_max, dtype=np.float64)
