@@ -62,11 +62,11 @@ __device__ __forceinline__ float bilerp(kk_float4u t, float fu, float fv) {
   return fmaf(fu, l1 - l0, l0);
 }
 
+// One workgroup's tile (16 x 16 columns of K voxels) of logical block id
+// `bid`.
 template <int K>
-__global__ __launch_bounds__(256) void cone_bp_kernel(
-    const kk::BpAngle* __restrict__ tab, BpArgs A) {
-  const int64_t bid =
-      kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group);
+__device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
+                                        const BpArgs& A, int64_t bid) {
   const int32_t tx = static_cast<int32_t>(bid % A.tiles_x);
   const int64_t t = bid / A.tiles_x;
   const int32_t ty = static_cast<int32_t>(t % A.tiles_y);
@@ -174,84 +174,106 @@ __global__ __launch_bounds__(256) void cone_bp_kernel(
   }
 }
 
-}  // namespace
+template <int K>
+__global__ __launch_bounds__(256) void cone_bp_kernel(
+    const kk::BpAngle* __restrict__ tab, BpArgs A) {
+  bp_tile<K>(tab, A, kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
+}
 
-namespace kk {
-
-struct BpEpilogue {
-  const float* C = nullptr;
-  const int64_t* cs = nullptr;
-  const float* mask = nullptr;
-  const int64_t* ms = nullptr;
-  float lo = 0.f, hi = 0.f;
-  int32_t clip_lo = 0, clip_hi = 0;
-  float* xtex = nullptr;  // FP frame textures of x (layout kk_internal.h)
-  int32_t xtex_mask = 0;
+struct BpItem {
+  const kk::BpAngle* tab;
+  BpArgs args;
 };
 
-int launch_cone_bp(const kk_plan* plan, const float* tex, int32_t n_angles,
-                   int32_t rows, int32_t cols, float* vol,
-                   const int32_t shape[3], const int64_t vs[3], float scale,
-                   int32_t mode, const BpEpilogue& ep, int32_t zrun,
-                   hipStream_t stream) {
-  if (plan == nullptr || !plan->bp_set)
+// Many problems, one persistent launch (see cone_fp_batch_kernel).
+template <int K>
+__global__ __launch_bounds__(256) void cone_bp_batch_kernel(
+    const BpItem* __restrict__ items, const int64_t* __restrict__ first,
+    int32_t n, int64_t total) {
+  for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (first[mid] <= g)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const BpItem& it = items[lo];
+    const int64_t local = g - first[lo];
+    bp_tile<K>(it.tab, it.args,
+               kk_xcd_remap(local, it.args.total_blocks, it.args.xcd_group));
+  }
+}
+
+}  // namespace
+
+namespace {
+
+// Validate one backprojection problem and fill its kernel arguments for
+// z-runs of `zrun` voxels.
+int make_bp_args(const kk::BpSpec& S, int32_t zrun, BpArgs& A) {
+  using namespace kk;
+  if (S.plan == nullptr || !S.plan->bp_set)
     return fail(KK_ERR_NOT_SET,
                 "Please set the parameters with `set_params()`.");
-  if (tex == nullptr || vol == nullptr || shape == nullptr || vs == nullptr)
+  if (S.tex == nullptr || S.out == nullptr)
     return fail(KK_ERR_INVALID, "kk_cone_bp: null argument");
-  if (n_angles != plan->bp_n)
+  if (S.n_angles != S.plan->bp_n)
     return fail(KK_ERR_SHAPE,
                 "kk_cone_bp: %d projections but parameters for %d angles",
-                n_angles, plan->bp_n);
-  if (rows <= 0 || cols <= 0 || shape[0] <= 0 || shape[1] <= 0 ||
-      shape[2] <= 0)
+                S.n_angles, S.plan->bp_n);
+  if (S.rows <= 0 || S.cols <= 0 || S.shape[0] <= 0 || S.shape[1] <= 0 ||
+      S.shape[2] <= 0)
     return fail(KK_ERR_INVALID, "kk_cone_bp: bad sizes");
-  const int64_t trb = int64_t(cols + kHalo) * 8;
-  const int64_t tab_b = int64_t(rows + kHalo) * trb;
+  if (zrun != 4 && zrun != 8 && zrun != 16)
+    return fail(KK_ERR_INVALID, "kk_cone_bp: z-run must be 4, 8 or 16");
+  const int64_t trb = int64_t(S.cols + kHalo) * 8;
+  const int64_t tab_b = int64_t(S.rows + kHalo) * trb;
   if (tab_b >= (int64_t(1) << 32) || trb >= (int64_t(1) << 24))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: detector too large");
-  if (mode == 2 && (ep.C == nullptr || ep.cs == nullptr))
+  if (S.mode == 2 && S.C == nullptr)
     return fail(KK_ERR_INVALID, "kk_cone_bp: SIRT update needs C");
-  BpArgs A{};
-  A.tex = reinterpret_cast<const char*>(tex);
+  A = BpArgs{};
+  A.tex = reinterpret_cast<const char*>(S.tex);
   A.tab_b = uint32_t(tab_b);
   A.trb = uint32_t(trb);
-  A.nx = shape[0];
-  A.ny = shape[1];
-  A.nz = shape[2];
-  A.n_angles = plan->bp_n;
-  A.rows = rows;
-  A.cols = cols;
-  A.out = vol;
-  A.vs0 = vs[0];
-  A.vs1 = vs[1];
-  A.vs2 = vs[2];
-  A.scale = scale;
-  A.mode = mode;
-  if (mode == 2) {
-    A.C = ep.C;
-    A.cs0 = ep.cs[0];
-    A.cs1 = ep.cs[1];
-    A.cs2 = ep.cs[2];
-    A.mask = ep.mask;
-    if (ep.mask) {
-      A.ms0 = ep.ms[0];
-      A.ms1 = ep.ms[1];
-      A.ms2 = ep.ms[2];
+  A.nx = S.shape[0];
+  A.ny = S.shape[1];
+  A.nz = S.shape[2];
+  A.n_angles = S.plan->bp_n;
+  A.rows = S.rows;
+  A.cols = S.cols;
+  A.out = S.out;
+  A.vs0 = S.vs[0];
+  A.vs1 = S.vs[1];
+  A.vs2 = S.vs[2];
+  A.scale = S.scale;
+  A.mode = S.mode;
+  if (S.mode == 2) {
+    A.C = S.C;
+    A.cs0 = S.cs[0];
+    A.cs1 = S.cs[1];
+    A.cs2 = S.cs[2];
+    A.mask = S.mask;
+    if (S.mask) {
+      A.ms0 = S.ms[0];
+      A.ms1 = S.ms[1];
+      A.ms2 = S.ms[2];
     }
-    A.lo = ep.lo;
-    A.hi = ep.hi;
-    A.clip_lo = ep.clip_lo;
-    A.clip_hi = ep.clip_hi;
-    A.xtex = ep.xtex;
-    if (ep.xtex) {
+    A.lo = S.lo;
+    A.hi = S.hi;
+    A.clip_lo = S.clip_lo;
+    A.clip_hi = S.clip_hi;
+    A.xtex = S.xtex;
+    if (S.xtex) {
       // voxel (x, y, z) in frame a: element (c_a, c_b + 1, c_c + 1)
       // component 0 and element (c_a, c_b, c_c + 1) component 1
       int64_t off = 0;  // floats
       for (int a = 0; a < 3; ++a) {
-        if (!(ep.xtex_mask & (1 << a))) continue;
+        if (!(S.xtex_mask & (1 << a))) continue;
         const int b = kFrameB[a], c = kFrameC[a];
-        const int64_t tcn = shape[c] + kHalo, tbn = shape[b] + kHalo;
+        const int64_t tcn = S.shape[c] + kHalo, tbn = S.shape[b] + kHalo;
         int64_t ws[3];
         ws[a] = 2 * tbn * tcn;
         ws[b] = 2 * tcn;
@@ -262,7 +284,7 @@ int launch_cone_bp(const kk_plan* plan, const float* tex, int32_t n_angles,
         A.fr_s1[f] = ws[1];
         A.fr_s2[f] = ws[2];
         A.fr_dual[f] = 1 - 2 * tcn;
-        off += 2 * frame_elems(shape, a);
+        off += 2 * frame_elems(S.shape, a);
       }
     }
   }
@@ -274,26 +296,187 @@ int launch_cone_bp(const kk_plan* plan, const float* tex, int32_t n_angles,
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: grid too large");
   const int32_t plane = A.tiles_x * A.tiles_y;
   A.xcd_group = (plane % kNumXcd == 0) ? plane / kNumXcd : 1;
+  return KK_OK;
+}
+
+int launch_bp_spec(const kk::BpSpec& S, int32_t zrun, hipStream_t stream) {
+  BpArgs A;
+  const int rc = make_bp_args(S, zrun, A);
+  if (rc != KK_OK) return rc;
   const dim3 grid(unsigned(A.total_blocks)), block(256);
   switch (zrun) {
     case 4:
-      hipLaunchKernelGGL(cone_bp_kernel<4>, grid, block, 0, stream, plan->bp,
-                         A);
+      hipLaunchKernelGGL(cone_bp_kernel<4>, grid, block, 0, stream,
+                         S.plan->bp, A);
       break;
     case 8:
-      hipLaunchKernelGGL(cone_bp_kernel<8>, grid, block, 0, stream, plan->bp,
-                         A);
-      break;
-    case 16:
-      hipLaunchKernelGGL(cone_bp_kernel<16>, grid, block, 0, stream,
-                         plan->bp, A);
+      hipLaunchKernelGGL(cone_bp_kernel<8>, grid, block, 0, stream,
+                         S.plan->bp, A);
       break;
     default:
-      return fail(KK_ERR_INVALID, "kk_cone_bp: z-run must be 4, 8 or 16");
+      hipLaunchKernelGGL(cone_bp_kernel<16>, grid, block, 0, stream,
+                         S.plan->bp, A);
+      break;
   }
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "cone_bp_kernel");
+  if (e != hipSuccess) return kk::hip_fail(e, "cone_bp_kernel");
   return KK_OK;
+}
+
+}  // namespace
+
+namespace kk {
+
+struct BpBatch {
+  BpItem* items = nullptr;
+  int64_t* first = nullptr;
+  int32_t n = 0;
+  int64_t total = 0;
+  int32_t zrun = 8;
+};
+
+int bp_batch_build(const std::vector<BpSpec>& specs, int32_t zrun,
+                   BpBatch** out) {
+  if (specs.empty()) return fail(KK_ERR_INVALID, "empty backprojection batch");
+  if (zrun != 4 && zrun != 8 && zrun != 16) zrun = 8;
+  std::vector<BpItem> items(specs.size());
+  std::vector<int64_t> first(specs.size());
+  int64_t total = 0;
+  for (size_t p = 0; p < specs.size(); ++p) {
+    const int rc = make_bp_args(specs[p], zrun, items[p].args);
+    if (rc != KK_OK) return rc;
+    items[p].tab = specs[p].plan->bp;
+    first[p] = total;
+    total += items[p].args.total_blocks;
+  }
+  auto* b = new (std::nothrow) BpBatch();
+  if (b == nullptr) return fail(KK_ERR_INVALID, "out of host memory");
+  b->n = static_cast<int32_t>(specs.size());
+  b->total = total;
+  b->zrun = zrun;
+  hipError_t e = hipMalloc(&b->items, sizeof(BpItem) * items.size());
+  if (e == hipSuccess) e = hipMalloc(&b->first, sizeof(int64_t) * first.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(b->items, items.data(), sizeof(BpItem) * items.size(),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->first, first.data(), sizeof(int64_t) * first.size(),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    bp_batch_free(b);
+    return hip_fail(e, "backprojection batch descriptors");
+  }
+  *out = b;
+  return KK_OK;
+}
+
+void bp_batch_free(BpBatch* b) {
+  if (b == nullptr) return;
+  if (b->items) hipFree(b->items);
+  if (b->first) hipFree(b->first);
+  delete b;
+}
+
+int bp_batch_launch(const BpBatch* b, int32_t grid_blocks,
+                    hipStream_t stream) {
+  if (b == nullptr) return fail(KK_ERR_NOT_SET, "backprojection batch not built");
+  int64_t g = grid_blocks > 0 ? grid_blocks : b->total;
+  if (g > b->total) g = b->total;
+  if (g <= 0) return KK_OK;
+  const dim3 grid(static_cast<unsigned>(g)), block(256);
+  switch (b->zrun) {
+    case 4:
+      hipLaunchKernelGGL(cone_bp_batch_kernel<4>, grid, block, 0, stream,
+                         b->items, b->first, b->n, b->total);
+      break;
+    case 8:
+      hipLaunchKernelGGL(cone_bp_batch_kernel<8>, grid, block, 0, stream,
+                         b->items, b->first, b->n, b->total);
+      break;
+    default:
+      hipLaunchKernelGGL(cone_bp_batch_kernel<16>, grid, block, 0, stream,
+                         b->items, b->first, b->n, b->total);
+      break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "cone_bp_batch_kernel");
+  return KK_OK;
+}
+
+}  // namespace kk
+
+namespace {
+
+int fill_bp_spec(kk::BpSpec& S, const kk_plan* plan, const float* tex,
+                 int32_t n_angles, int32_t rows, int32_t cols, float* vol,
+                 const int32_t shape[3], const int64_t vs[3], float scale,
+                 int32_t mode) {
+  if (shape == nullptr || vs == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_cone_bp: null argument");
+  S.plan = plan;
+  S.tex = tex;
+  S.n_angles = n_angles;
+  S.rows = rows;
+  S.cols = cols;
+  S.out = vol;
+  for (int t = 0; t < 3; ++t) {
+    S.shape[t] = shape[t];
+    S.vs[t] = vs[t];
+  }
+  S.scale = scale;
+  S.mode = mode;
+  return KK_OK;
+}
+
+int fill_sirt_update(kk::BpSpec& S, const float* C, const int64_t cs[3],
+                     const float* mask, const int64_t ms[3], float lo,
+                     float hi, int32_t clip_lo, int32_t clip_hi, float* x_tex,
+                     int32_t x_tex_mask) {
+  if (C == nullptr || cs == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_sirt_bp_update: C is required");
+  if (mask != nullptr && ms == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_sirt_bp_update: mask strides");
+  if (x_tex != nullptr && (x_tex_mask <= 0 || x_tex_mask > 7))
+    return kk::fail(KK_ERR_INVALID, "kk_sirt_bp_update: texture mask");
+  S.C = C;
+  S.mask = mask;
+  for (int t = 0; t < 3; ++t) {
+    S.cs[t] = cs[t];
+    S.ms[t] = mask ? ms[t] : 0;
+  }
+  S.lo = lo;
+  S.hi = hi;
+  S.clip_lo = clip_lo;
+  S.clip_hi = clip_hi;
+  S.xtex = x_tex;
+  S.xtex_mask = x_tex ? x_tex_mask : 0;
+  return KK_OK;
+}
+
+}  // namespace
+
+namespace kk {
+
+int bp_spec(BpSpec& S, const kk_plan* plan, const float* tex,
+            int32_t n_angles, int32_t rows, int32_t cols, float* vol,
+            const int32_t shape[3], const int64_t vs[3], float scale,
+            int32_t additive) {
+  return fill_bp_spec(S, plan, tex, n_angles, rows, cols, vol, shape, vs,
+                      scale, additive ? 1 : 0);
+}
+
+int sirt_update_spec(BpSpec& S, const kk_plan* plan, const float* res_tex,
+                     int32_t n_angles, int32_t rows, int32_t cols, float* x,
+                     const int32_t shape[3], const int64_t xs[3],
+                     const float* C, const int64_t cs[3], const float* mask,
+                     const int64_t ms[3], float lo, float hi,
+                     int32_t clip_lo, int32_t clip_hi, float* x_tex,
+                     int32_t x_tex_mask, float scale) {
+  int rc = fill_bp_spec(S, plan, res_tex, n_angles, rows, cols, x, shape, xs,
+                        scale, 2);
+  if (rc != KK_OK) return rc;
+  return fill_sirt_update(S, C, cs, mask, ms, lo, hi, clip_lo, clip_hi,
+                          x_tex, x_tex_mask);
 }
 
 }  // namespace kk
@@ -303,10 +486,11 @@ extern "C" int kk_cone_bp(const kk_plan* plan, const float* proj_tex,
                           float* vol, const int32_t shape[3],
                           const int64_t vs[3], float output_scale,
                           int32_t additive, void* stream) {
-  kk::BpEpilogue ep;
-  return kk::launch_cone_bp(plan, proj_tex, n_angles, rows, cols, vol, shape,
-                            vs, output_scale, additive ? 1 : 0, ep, 8,
-                            static_cast<hipStream_t>(stream));
+  kk::BpSpec S;
+  const int rc = kk::bp_spec(S, plan, proj_tex, n_angles, rows, cols, vol,
+                             shape, vs, output_scale, additive);
+  if (rc != KK_OK) return rc;
+  return launch_bp_spec(S, 8, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int kk_sirt_bp_update(const kk_plan* plan, const float* res_tex,
@@ -319,22 +503,11 @@ extern "C" int kk_sirt_bp_update(const kk_plan* plan, const float* res_tex,
                                  int32_t clip_hi, float* x_tex,
                                  int32_t x_tex_mask, float output_scale,
                                  void* stream) {
-  kk::BpEpilogue ep;
-  ep.C = C;
-  ep.cs = cs;
-  ep.mask = mask;
-  ep.ms = ms;
-  if (mask != nullptr && ms == nullptr)
-    return kk::fail(KK_ERR_INVALID, "kk_sirt_bp_update: mask strides");
-  if (x_tex != nullptr && (x_tex_mask <= 0 || x_tex_mask > 7))
-    return kk::fail(KK_ERR_INVALID, "kk_sirt_bp_update: texture mask");
-  ep.lo = lo;
-  ep.hi = hi;
-  ep.clip_lo = clip_lo;
-  ep.clip_hi = clip_hi;
-  ep.xtex = x_tex;
-  ep.xtex_mask = x_tex_mask;
-  return kk::launch_cone_bp(plan, res_tex, n_angles, rows, cols, x, shape,
-                            xs, output_scale, 2, ep, 8,
-                            static_cast<hipStream_t>(stream));
+  kk::BpSpec S;
+  const int rc = kk::sirt_update_spec(S, plan, res_tex, n_angles, rows, cols,
+                                      x, shape, xs, C, cs, mask, ms, lo, hi,
+                                      clip_lo, clip_hi, x_tex, x_tex_mask,
+                                      output_scale);
+  if (rc != KK_OK) return rc;
+  return launch_bp_spec(S, 8, static_cast<hipStream_t>(stream));
 }

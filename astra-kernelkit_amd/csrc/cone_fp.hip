@@ -34,6 +34,9 @@
 // * per-angle parameters are wave-uniform (scalar loads); workgroups are
 //   ordered row band -> angle -> column tile, and grouped so that each XCD
 //   keeps the same column range across consecutive angles (L2 reuse).
+#include <new>
+#include <vector>
+
 #include "kk_internal.h"
 
 namespace {
@@ -128,11 +131,11 @@ __device__ __forceinline__ void trace(const kk::FpAngle& P,
   }
 }
 
+// One workgroup's tile (row band rt, angle, column tile ct) of logical
+// block id `bid`.
 template <int R>
-__global__ __launch_bounds__(256) void cone_fp_kernel(
-    const kk::FpAngle* __restrict__ tab, FpArgs A) {
-  const int64_t bid =
-      kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group);
+__device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
+                                        const FpArgs& A, int64_t bid) {
   const int32_t ct = static_cast<int32_t>(bid % A.col_tiles);
   const int64_t t = bid / A.col_tiles;
   const int32_t ang = static_cast<int32_t>(t % A.n_angles);
@@ -214,85 +217,269 @@ __global__ __launch_bounds__(256) void cone_fp_kernel(
   }
 }
 
+template <int R>
+__global__ __launch_bounds__(256) void cone_fp_kernel(
+    const kk::FpAngle* __restrict__ tab, FpArgs A) {
+  fp_tile<R>(tab, A, kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
+}
+
+}  // namespace
+
+namespace {
+
+int auto_rows_per_thread(int32_t rows, int32_t rpt) {
+  if (rpt == 1 || rpt == 2 || rpt == 4) return rpt;
+  return rows > 128 ? 4 : (rows > 64 ? 2 : 1);
+}
+
+// Validate one forward-projection problem and fill its kernel arguments
+// for `rpt` rows per thread.
+int make_fp_args(const kk::FpSpec& S, int32_t rpt, FpArgs& A) {
+  using kk::fail;
+  if (S.plan == nullptr || !S.plan->fp_set)
+    return fail(KK_ERR_NOT_SET,
+                "Please set the projection geometry with `set_params()`.");
+  if (S.tex == nullptr || S.out == nullptr)
+    return fail(KK_ERR_INVALID, "kk_cone_fp: null argument");
+  if (S.rows <= 0 || S.cols <= 0)
+    return fail(KK_ERR_INVALID, "kk_cone_fp: bad detector size");
+  if (S.mode == 2 && (S.y == nullptr || S.R == nullptr))
+    return fail(KK_ERR_INVALID, "kk_cone_fp: residual mode needs y and R");
+  if (S.mode == 1 && S.out_dual != 0)
+    return fail(KK_ERR_INVALID, "kk_cone_fp: dual output is write-only");
+  A = FpArgs{};
+  A.tex = reinterpret_cast<const char*>(S.tex);
+  A.out = S.out;
+  A.os0 = S.os[0];
+  A.os1 = S.os[1];
+  A.os2 = S.os[2];
+  A.out_dual = S.out_dual;
+  if (S.mode == 2) {
+    A.y = S.y;
+    A.ys0 = S.ys[0];
+    A.ys1 = S.ys[1];
+    A.ys2 = S.ys[2];
+    A.R = S.R;
+    A.rs0 = S.rs[0];
+    A.rs1 = S.rs[1];
+    A.rs2 = S.rs[2];
+  }
+  A.rows = S.rows;
+  A.cols = S.cols;
+  A.n_angles = S.plan->fp_n;
+  A.row_tiles = (S.rows + kWave * rpt - 1) / (kWave * rpt);
+  A.col_tiles = (S.cols + kColsPerBlock - 1) / kColsPerBlock;
+  A.mode = S.mode;
+  A.total_blocks = int64_t(A.row_tiles) * A.n_angles * A.col_tiles;
+  if (A.total_blocks >= (int64_t(1) << 31))
+    return fail(KK_ERR_UNSUPPORTED, "kk_cone_fp: grid too large");
+  A.xcd_group = (A.col_tiles % kk::kNumXcd == 0) ? A.col_tiles / kk::kNumXcd
+                                                  : 1;
+  return KK_OK;
+}
+
+// Many problems, one launch: a persistent grid walks the concatenated
+// workgroup tiles of all problems; a tile finds its problem by binary
+// search over the problems' first tile ids (wave-uniform).
+struct FpItem {
+  const kk::FpAngle* tab;
+  FpArgs args;
+};
+
+template <int R>
+__global__ __launch_bounds__(256) void cone_fp_batch_kernel(
+    const FpItem* __restrict__ items, const int64_t* __restrict__ first,
+    int32_t n, int64_t total) {
+  for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+    int32_t lo = 0, hi = n - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (first[mid] <= g)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const FpItem& it = items[lo];
+    const int64_t local = g - first[lo];
+    fp_tile<R>(it.tab, it.args,
+               kk_xcd_remap(local, it.args.total_blocks, it.args.xcd_group));
+  }
+}
+
 }  // namespace
 
 namespace kk {
 
-int launch_cone_fp(const kk_plan* plan, const float* tex, float* out,
-                   const int64_t os[3], int64_t out_dual, int32_t rows,
-                   int32_t cols, int32_t mode, const float* y,
-                   const int64_t ys[3], const float* R, const int64_t rs[3],
-                   int32_t rows_per_thread, hipStream_t stream) {
-  if (plan == nullptr || !plan->fp_set)
-    return fail(KK_ERR_NOT_SET,
-                "Please set the projection geometry with `set_params()`.");
-  if (tex == nullptr || out == nullptr || os == nullptr)
-    return fail(KK_ERR_INVALID, "kk_cone_fp: null argument");
-  if (rows <= 0 || cols <= 0)
-    return fail(KK_ERR_INVALID, "kk_cone_fp: bad detector size");
-  if (mode == 2 && (y == nullptr || R == nullptr || ys == nullptr ||
-                    rs == nullptr))
-    return fail(KK_ERR_INVALID, "kk_cone_fp: residual mode needs y and R");
-  if (mode == 1 && out_dual != 0)
-    return fail(KK_ERR_INVALID, "kk_cone_fp: dual output is write-only");
-  if (rows_per_thread != 1 && rows_per_thread != 2 && rows_per_thread != 4)
-    rows_per_thread = rows > 128 ? 4 : (rows > 64 ? 2 : 1);
-  FpArgs A{};
-  A.tex = reinterpret_cast<const char*>(tex);
-  A.out = out;
-  A.os0 = os[0];
-  A.os1 = os[1];
-  A.os2 = os[2];
-  A.out_dual = out_dual;
-  if (mode == 2) {
-    A.y = y;
-    A.ys0 = ys[0];
-    A.ys1 = ys[1];
-    A.ys2 = ys[2];
-    A.R = R;
-    A.rs0 = rs[0];
-    A.rs1 = rs[1];
-    A.rs2 = rs[2];
+struct FpBatch {
+  FpItem* items = nullptr;
+  int64_t* first = nullptr;
+  int32_t n = 0;
+  int64_t total = 0;
+  int32_t rpt = 4;
+};
+
+int fp_batch_build(const std::vector<FpSpec>& specs, int32_t rows_per_thread,
+                   FpBatch** out) {
+  if (specs.empty()) return fail(KK_ERR_INVALID, "empty forward batch");
+  int32_t max_rows = 0;
+  for (const FpSpec& S : specs) max_rows = S.rows > max_rows ? S.rows : max_rows;
+  const int32_t rpt = auto_rows_per_thread(max_rows, rows_per_thread);
+  std::vector<FpItem> items(specs.size());
+  std::vector<int64_t> first(specs.size());
+  int64_t total = 0;
+  for (size_t p = 0; p < specs.size(); ++p) {
+    const int rc = make_fp_args(specs[p], rpt, items[p].args);
+    if (rc != KK_OK) return rc;
+    items[p].tab = specs[p].plan->fp;
+    first[p] = total;
+    total += items[p].args.total_blocks;
   }
-  A.rows = rows;
-  A.cols = cols;
-  A.n_angles = plan->fp_n;
-  A.row_tiles = (rows + kWave * rows_per_thread - 1) /
-                (kWave * rows_per_thread);
-  A.col_tiles = (cols + kColsPerBlock - 1) / kColsPerBlock;
-  A.mode = mode;
-  A.total_blocks = int64_t(A.row_tiles) * A.n_angles * A.col_tiles;
-  if (A.total_blocks >= (int64_t(1) << 31))
-    return fail(KK_ERR_UNSUPPORTED, "kk_cone_fp: grid too large");
-  A.xcd_group = (A.col_tiles % kNumXcd == 0) ? A.col_tiles / kNumXcd : 1;
-  const dim3 grid(unsigned(A.total_blocks)), block(256);
-  switch (rows_per_thread) {
+  auto* b = new (std::nothrow) FpBatch();
+  if (b == nullptr) return fail(KK_ERR_INVALID, "out of host memory");
+  b->n = static_cast<int32_t>(specs.size());
+  b->total = total;
+  b->rpt = rpt;
+  hipError_t e = hipMalloc(&b->items, sizeof(FpItem) * items.size());
+  if (e == hipSuccess) e = hipMalloc(&b->first, sizeof(int64_t) * first.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(b->items, items.data(), sizeof(FpItem) * items.size(),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(b->first, first.data(), sizeof(int64_t) * first.size(),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    fp_batch_free(b);
+    return hip_fail(e, "forward batch descriptors");
+  }
+  *out = b;
+  return KK_OK;
+}
+
+void fp_batch_free(FpBatch* b) {
+  if (b == nullptr) return;
+  if (b->items) hipFree(b->items);
+  if (b->first) hipFree(b->first);
+  delete b;
+}
+
+int fp_batch_launch(const FpBatch* b, int32_t grid_blocks,
+                    hipStream_t stream) {
+  if (b == nullptr) return fail(KK_ERR_NOT_SET, "forward batch not built");
+  int64_t g = grid_blocks > 0 ? grid_blocks : b->total;
+  if (g > b->total) g = b->total;
+  if (g <= 0) return KK_OK;
+  const dim3 grid(static_cast<unsigned>(g)), block(256);
+  switch (b->rpt) {
     case 1:
-      hipLaunchKernelGGL(cone_fp_kernel<1>, grid, block, 0, stream, plan->fp,
-                         A);
+      hipLaunchKernelGGL(cone_fp_batch_kernel<1>, grid, block, 0, stream,
+                         b->items, b->first, b->n, b->total);
       break;
     case 2:
-      hipLaunchKernelGGL(cone_fp_kernel<2>, grid, block, 0, stream, plan->fp,
-                         A);
+      hipLaunchKernelGGL(cone_fp_batch_kernel<2>, grid, block, 0, stream,
+                         b->items, b->first, b->n, b->total);
       break;
     default:
-      hipLaunchKernelGGL(cone_fp_kernel<4>, grid, block, 0, stream, plan->fp,
-                         A);
+      hipLaunchKernelGGL(cone_fp_batch_kernel<4>, grid, block, 0, stream,
+                         b->items, b->first, b->n, b->total);
       break;
   }
   hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return hip_fail(e, "cone_fp_kernel");
+  if (e != hipSuccess) return hip_fail(e, "cone_fp_batch_kernel");
   return KK_OK;
 }
 
 }  // namespace kk
 
+namespace kk {
+
+int fp_spec(FpSpec& S, const kk_plan* plan, const float* tex, float* out,
+            const int64_t os[3], int32_t rows, int32_t cols,
+            int32_t additive) {
+  if (os == nullptr) return fail(KK_ERR_INVALID, "kk_cone_fp: null strides");
+  S = FpSpec{};
+  S.plan = plan;
+  S.tex = tex;
+  S.out = out;
+  for (int t = 0; t < 3; ++t) S.os[t] = os[t];
+  S.rows = rows;
+  S.cols = cols;
+  S.mode = additive ? 1 : 0;
+  return KK_OK;
+}
+
+int sirt_residual_spec(FpSpec& S, const kk_plan* plan, const float* x_tex,
+                       const float* y, const int64_t ys[3], const float* R,
+                       const int64_t rs[3], float* res_tex, int32_t n_angles,
+                       int32_t rows, int32_t cols) {
+  if (plan == nullptr || !plan->fp_set)
+    return fail(KK_ERR_NOT_SET, "kk_sirt_fp_residual: no parameters");
+  if (n_angles != plan->fp_n)
+    return fail(KK_ERR_SHAPE, "kk_sirt_fp_residual: angle count");
+  if (res_tex == nullptr || ys == nullptr || rs == nullptr)
+    return fail(KK_ERR_INVALID, "kk_sirt_fp_residual: null argument");
+  // write the residual straight into the backprojection texture: P(a, r, c)
+  // is component 0 of element (a, r+1, c+1) and component 1 of (a, r, c+1)
+  const int64_t tc = cols + kHalo, tr = rows + kHalo;
+  S = FpSpec{};
+  S.plan = plan;
+  S.tex = x_tex;
+  S.out = res_tex + 2 * (kHaloLo * tc + kHaloLo);
+  S.os[0] = 2 * tr * tc;
+  S.os[1] = 2 * tc;
+  S.os[2] = 2;
+  S.out_dual = 1 - 2 * tc;
+  S.rows = rows;
+  S.cols = cols;
+  S.mode = 2;
+  S.y = y;
+  S.R = R;
+  for (int t = 0; t < 3; ++t) {
+    S.ys[t] = ys[t];
+    S.rs[t] = rs[t];
+  }
+  return KK_OK;
+}
+
+}  // namespace kk
+
+namespace {
+
+int launch_fp_spec(const kk::FpSpec& S, int32_t rows_per_thread,
+                   hipStream_t stream) {
+  const int32_t rpt = auto_rows_per_thread(S.rows, rows_per_thread);
+  FpArgs A;
+  const int rc = make_fp_args(S, rpt, A);
+  if (rc != KK_OK) return rc;
+  const dim3 grid(unsigned(A.total_blocks)), block(256);
+  switch (rpt) {
+    case 1:
+      hipLaunchKernelGGL(cone_fp_kernel<1>, grid, block, 0, stream,
+                         S.plan->fp, A);
+      break;
+    case 2:
+      hipLaunchKernelGGL(cone_fp_kernel<2>, grid, block, 0, stream,
+                         S.plan->fp, A);
+      break;
+    default:
+      hipLaunchKernelGGL(cone_fp_kernel<4>, grid, block, 0, stream,
+                         S.plan->fp, A);
+      break;
+  }
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_kernel");
+  return KK_OK;
+}
+
+}  // namespace
+
 extern "C" int kk_cone_fp(const kk_plan* plan, const float* vol_tex,
                           float* proj, const int64_t ps[3], int32_t rows,
                           int32_t cols, int32_t additive, void* stream) {
-  return kk::launch_cone_fp(plan, vol_tex, proj, ps, 0, rows, cols,
-                            additive ? 1 : 0, nullptr, nullptr, nullptr,
-                            nullptr, 0, static_cast<hipStream_t>(stream));
+  kk::FpSpec S;
+  const int rc = kk::fp_spec(S, plan, vol_tex, proj, ps, rows, cols,
+                             additive);
+  if (rc != KK_OK) return rc;
+  return launch_fp_spec(S, 0, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
@@ -300,18 +487,9 @@ extern "C" int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
                                    const float* R, const int64_t rs[3],
                                    float* res_tex, int32_t n_angles,
                                    int32_t rows, int32_t cols, void* stream) {
-  if (plan == nullptr || !plan->fp_set)
-    return kk::fail(KK_ERR_NOT_SET, "kk_sirt_fp_residual: no parameters");
-  if (n_angles != plan->fp_n)
-    return kk::fail(KK_ERR_SHAPE, "kk_sirt_fp_residual: angle count");
-  if (res_tex == nullptr)
-    return kk::fail(KK_ERR_INVALID, "kk_sirt_fp_residual: null texture");
-  // write the residual straight into the backprojection texture: P(a, r, c)
-  // is component 0 of element (a, r+1, c+1) and component 1 of (a, r, c+1)
-  const int64_t tc = cols + kk::kHalo, tr = rows + kk::kHalo;
-  const int64_t os[3] = {2 * tr * tc, 2 * tc, 2};
-  float* out = res_tex + 2 * (kk::kHaloLo * tc + kk::kHaloLo);
-  return kk::launch_cone_fp(plan, x_tex, out, os, 1 - 2 * tc, rows, cols, 2,
-                            y, ys, R, rs, 0,
-                            static_cast<hipStream_t>(stream));
+  kk::FpSpec S;
+  const int rc = kk::sirt_residual_spec(S, plan, x_tex, y, ys, R, rs,
+                                        res_tex, n_angles, rows, cols);
+  if (rc != KK_OK) return rc;
+  return launch_fp_spec(S, 0, static_cast<hipStream_t>(stream));
 }

@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 #include "kernelkit_hip.h"
 
@@ -81,6 +82,78 @@ struct alignas(16) BpAngle {
 };
 static_assert(sizeof(BpAngle) == 64, "BpAngle layout");
 
+// One forward-projection problem of a launch: the plan's table read from
+// the frame texture `tex`, written to `out` (element strides os) as
+// mode 0: p, 1: += p, 2: (p - y) * R (SIRT residual), with an optional
+// second write at out + out_dual.
+struct FpSpec {
+  const kk_plan_s* plan = nullptr;
+  const float* tex = nullptr;
+  float* out = nullptr;
+  int64_t os[3] = {0, 0, 0};
+  int64_t out_dual = 0;
+  int32_t rows = 0, cols = 0, mode = 0;
+  const float* y = nullptr;
+  int64_t ys[3] = {0, 0, 0};
+  const float* R = nullptr;
+  int64_t rs[3] = {0, 0, 0};
+};
+
+// One backprojection problem: mode 0: b, 1: += b, 2: the SIRT update
+// x <- clip(mask * (x - C * b)) with the FP frame textures of x refreshed.
+struct BpSpec {
+  const kk_plan_s* plan = nullptr;
+  const float* tex = nullptr;
+  int32_t n_angles = 0, rows = 0, cols = 0;
+  float* out = nullptr;
+  int32_t shape[3] = {0, 0, 0};
+  int64_t vs[3] = {0, 0, 0};
+  float scale = 1.f;
+  int32_t mode = 0;
+  const float* C = nullptr;
+  int64_t cs[3] = {0, 0, 0};
+  const float* mask = nullptr;
+  int64_t ms[3] = {0, 0, 0};
+  float lo = 0.f, hi = 0.f;
+  int32_t clip_lo = 0, clip_hi = 0;
+  float* xtex = nullptr;
+  int32_t xtex_mask = 0;
+};
+
+// Device descriptors of many problems for one persistent launch.
+struct FpBatch;
+struct BpBatch;
+int fp_batch_build(const std::vector<FpSpec>& specs, int32_t rows_per_thread,
+                   FpBatch** out);
+void fp_batch_free(FpBatch* b);
+int fp_batch_launch(const FpBatch* b, int32_t grid_blocks, hipStream_t s);
+int bp_batch_build(const std::vector<BpSpec>& specs, int32_t zrun,
+                   BpBatch** out);
+void bp_batch_free(BpBatch* b);
+int bp_batch_launch(const BpBatch* b, int32_t grid_blocks, hipStream_t s);
+
+// Fill a BpSpec from the C-ABI arguments of kk_cone_bp / kk_sirt_bp_update
+// (validated when the launch arguments are built).
+int bp_spec(BpSpec& S, const kk_plan_s* plan, const float* tex,
+            int32_t n_angles, int32_t rows, int32_t cols, float* vol,
+            const int32_t shape[3], const int64_t vs[3], float scale,
+            int32_t additive);
+int sirt_update_spec(BpSpec& S, const kk_plan_s* plan, const float* res_tex,
+                     int32_t n_angles, int32_t rows, int32_t cols, float* x,
+                     const int32_t shape[3], const int64_t xs[3],
+                     const float* C, const int64_t cs[3], const float* mask,
+                     const int64_t ms[3], float lo, float hi,
+                     int32_t clip_lo, int32_t clip_hi, float* x_tex,
+                     int32_t x_tex_mask, float scale);
+// FpSpec of kk_cone_fp / kk_sirt_fp_residual.
+int fp_spec(FpSpec& S, const kk_plan_s* plan, const float* tex, float* out,
+            const int64_t os[3], int32_t rows, int32_t cols,
+            int32_t additive);
+int sirt_residual_spec(FpSpec& S, const kk_plan_s* plan, const float* x_tex,
+                       const float* y, const int64_t ys[3], const float* R,
+                       const int64_t rs[3], float* res_tex, int32_t n_angles,
+                       int32_t rows, int32_t cols);
+
 }  // namespace kk
 
 struct kk_plan_s {
@@ -91,6 +164,10 @@ struct kk_plan_s {
   int32_t fp_shape[3] = {0, 0, 0};
   int32_t fp_mask = 0;  // bit a: frame texture of ray axis a is used
   bool fp_set = false;
+  // angle ids grouped by ray axis: fr_idx[fr_start[a] .. fr_start[a+1])
+  int32_t* fr_idx = nullptr;
+  int32_t fr_cap = 0;
+  int32_t fr_start[4] = {0, 0, 0, 0};
   // backprojection table
   kk::BpAngle* bp = nullptr;
   int32_t bp_n = 0, bp_cap = 0;

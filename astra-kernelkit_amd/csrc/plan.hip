@@ -64,6 +64,7 @@ extern "C" int kk_plan_destroy(kk_plan* plan) {
   hipSetDevice(plan->device);
   if (plan->fp) hipFree(plan->fp);
   if (plan->bp) hipFree(plan->bp);
+  if (plan->fr_idx) hipFree(plan->fr_idx);
   hipSetDevice(dev);
   delete plan;
   return KK_OK;
@@ -227,9 +228,21 @@ extern "C" int kk_set_params_fp(kk_plan* plan, const double* params,
   int dev = 0;
   KK_HIP(hipGetDevice(&dev));
   KK_HIP(hipSetDevice(plan->device));
+  // angle ids grouped by ray axis (the transpose runs one pass per axis)
+  std::vector<int32_t> idx;
+  idx.reserve(n);
+  int32_t start[4] = {0, 0, 0, 0};
+  for (int a = 0; a < 3; ++a) {
+    start[a] = static_cast<int32_t>(idx.size());
+    for (int32_t k = 0; k < n; ++k)
+      if (tab[k].axis == a) idx.push_back(k);
+  }
+  start[3] = n;
   int rc = upload_table(&plan->fp, &plan->fp_cap, tab);
+  if (rc == KK_OK) rc = upload_table(&plan->fr_idx, &plan->fr_cap, idx);
   hipSetDevice(dev);
   if (rc != KK_OK) return rc;
+  for (int a = 0; a < 4; ++a) plan->fr_start[a] = start[a];
   plan->fp_n = n;
   for (int t = 0; t < 3; ++t) plan->fp_shape[t] = shape[t];
   plan->fp_mask = mask;

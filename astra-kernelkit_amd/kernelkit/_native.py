@@ -68,6 +68,30 @@ _SIGNATURES = {
     "kk_fill": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_float, _vp]),
     "kk_cone_fp": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, ctypes.c_int32,
                                   ctypes.c_int32, ctypes.c_int32, _vp]),
+    "kk_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "kk_batch_destroy": (ctypes.c_int, [_vp]),
+    "kk_batch_size": (ctypes.c_int32, [_vp, ctypes.c_int32]),
+    "kk_batch_add_fp": (ctypes.c_int, [
+        _vp, _vp, _vp, _vp, _c_i64p, ctypes.c_int32, ctypes.c_int32,
+        ctypes.c_int32]),
+    "kk_batch_add_bp": (ctypes.c_int, [
+        _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
+        _c_i32p, _c_i64p, ctypes.c_float, ctypes.c_int32]),
+    "kk_batch_add_sirt_fp": (ctypes.c_int, [
+        _vp, _vp, _vp, _vp, _c_i64p, _vp, _c_i64p, _vp, ctypes.c_int32,
+        ctypes.c_int32, ctypes.c_int32]),
+    "kk_batch_add_sirt_bp": (ctypes.c_int, [
+        _vp, _vp, _vp, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, _vp,
+        _c_i32p, _c_i64p, _vp, _c_i64p, _vp, _c_i64p, ctypes.c_float,
+        ctypes.c_float, ctypes.c_int32, ctypes.c_int32, _vp, ctypes.c_int32,
+        ctypes.c_float]),
+    "kk_batch_configure": (ctypes.c_int, [_vp, ctypes.c_int32,
+                                          ctypes.c_int32]),
+    "kk_batch_fp": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "kk_batch_bp": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "kk_cone_fp_transpose": (ctypes.c_int, [
+        _vp, _vp, _c_i64p, ctypes.c_int32, ctypes.c_int32, _vp, _c_i32p,
+        _c_i64p, ctypes.c_int32, _vp]),
     "kk_cone_bp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
                                   ctypes.c_int32, _vp, _c_i32p, _c_i64p,
                                   ctypes.c_float, ctypes.c_int32, _vp]),
@@ -204,6 +228,29 @@ class Plan:
         if ptr is not None and ptr.value and _lib is not None:
             try:
                 _lib.kk_plan_destroy(ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self._ptr = None
+
+
+class Batch:
+    """Owner of a library batch (problem descriptors for one launch per
+    operator)."""
+
+    def __init__(self):
+        ptr = _vp()
+        check(lib().kk_batch_create(ctypes.byref(ptr)))
+        self._ptr = ptr
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._ptr
+
+    def __del__(self):
+        ptr = getattr(self, "_ptr", None)
+        if ptr is not None and ptr.value and _lib is not None:
+            try:
+                _lib.kk_batch_destroy(ptr)
             except Exception:  # interpreter shutdown
                 pass
             self._ptr = None

@@ -326,3 +326,55 @@ class RayDrivenConeFP(BaseKernel):
             self._plan.ptr, _native._vp(tex.ptr), _native._vp(proj.data_ptr()),
             _native.i64(strides), rows, cols, 1 if additive else 0,
             _native.stream_ptr(stream)))
+
+    def transpose(self, projections, projection_geometry,
+                  volume_geometry: VolumeGeometry, volume, *,
+                  additive: bool = False, stream=None):
+        """Exact transpose of this forward projection (matched adjoint):
+        ``volume (+)= A^T projections``.
+
+        No counterpart in the reference (its gradients use the voxel-driven
+        backprojector, ``torch_support.py:29-33``).  ``projections`` is a
+        device tensor in ``projection_axes`` order, ``volume`` one in
+        ``volume_axes`` order.  Every (ray, slice) sample of :meth:`__call__`
+        adds its weighted value to its bilinear taps, with the forward
+        kernel's float32 arithmetic, so ``<A x, y> = <x, A^T y>`` up to
+        rounding.
+        """
+        if not self.is_compiled:
+            raise KernelNotCompiledException(
+                "Please compile the kernel with `compile()`.")
+        if not self._params_are_set:
+            raise KernelMemoryUnpreparedException(
+                "Please set the projection geometry with `set_params()`.")
+        import torch
+        for name, t in (("projections", projections), ("volume", volume)):
+            if not is_device_array(t):
+                raise TypeError(f"`{name}` must be a device tensor.")
+            if t.dtype != torch.float32:
+                raise NotImplementedError(
+                    "Currently there is only support for dtype float32, but "
+                    f"got {t.dtype}.")
+        n = self._params.shape[1]
+        rows, cols = detector_shape(projection_geometry)
+        logical = (n, rows, cols)
+        expect = tuple(logical[i] for i in self._proj_axs)
+        if tuple(projections.shape) != expect:
+            raise ValueError(
+                f"Projections shape {tuple(projections.shape)} does not match "
+                f"{expect} (angles, rows, cols permuted by "
+                f"{self._proj_axs}).")
+        world = tuple(int(s) for s in volume_geometry.shape)
+        expect_vol = tuple(world[i] for i in self._vol_axs)
+        if tuple(volume.shape) != expect_vol:
+            raise ValueError(
+                f"Volume shape {tuple(volume.shape)} does not match volume "
+                f"geometry {world} with axes {self._vol_axs}.")
+        self._upload(rows, cols, volume_geometry)
+        pst = _logical_strides(projections, self._proj_axs)
+        vst = _logical_strides(volume, self._vol_axs)
+        _native.check(_native.lib().kk_cone_fp_transpose(
+            self._plan.ptr, _native._vp(projections.data_ptr()),
+            _native.i64(pst), rows, cols, _native._vp(volume.data_ptr()),
+            _native.i32(world), _native.i64(vst), 1 if additive else 0,
+            _native.stream_ptr(stream)))

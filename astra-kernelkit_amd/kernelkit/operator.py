@@ -112,6 +112,45 @@ class ProjectorOperator(BaseOperator, ABC):
             self._T = _Adjoint()
         return self._T
 
+    @property
+    def matched_T(self):
+        """``A^T : Y -> X``, the exact transpose of the ray-driven forward
+        projection (no counterpart in the reference, whose ``.T`` is the
+        unmatched voxel-driven backprojector).  ``<A x, y> = <x, A^T y>``
+        up to float32 rounding; used for exact gradients."""
+        outer = self
+
+        class _MatchedTranspose(BaseOperator):
+            @property
+            def domain_shape(self) -> tuple:
+                return outer.range_shape
+
+            @property
+            def range_shape(self) -> tuple:
+                return outer.domain_shape
+
+            def __call__(self, input, out=None, additive=False):
+                if tuple(input.shape) != self.domain_shape:
+                    raise ValueError(
+                        f"Input shape {tuple(input.shape)} does not match "
+                        f"domain shape {self.domain_shape} of operator.")
+                if out is None:
+                    out = _zeros(self.range_shape)
+                p = outer.projector
+                p._require_geometry()
+                p.kernel.transpose(input, p._proj_geom_seq,
+                                   p.volume_geometry, out,
+                                   additive=additive)
+                return out
+
+            @property
+            def T(self):
+                return outer
+
+        if not hasattr(self, "_matched_T"):
+            self._matched_T = _MatchedTranspose()
+        return self._matched_T
+
 
 class XrayTransform(ProjectorOperator):
     """The cone-beam X-ray transform (``operator.py:149-233``)."""

@@ -152,3 +152,44 @@ def config_geometry(name: str, n_angles: int | None = None, seed: int = 0):
     pgs = circular_geometry(n, seed=seed, **c)
     vg = resolve_volume_geometry(shape=shape, voxel_size=1.0)
     return pgs, vg
+
+
+_BATCH_SIZES = (64, 96, 128, 160, 192, 224, 256)
+
+
+def batch_problems(n: int = 64, seed: int = 0, scale: float = 1.0) -> list:
+    """``n`` independent cone problems of uneven size, as
+    ``(projection geometries, volume geometry)`` pairs.
+
+    Volumes are cubic or anisotropic (``a x a x b``) with sides from 64 to
+    256 voxels (times ``scale``); 90-360 angles; source-origin distance
+    uniform in [300, 800] and magnification in [1.2, 2.5]; detectors of
+    96-384 pixels per side sized to the magnified volume; per-angle detector
+    tilt of up to 2 degrees and shifts of up to 5 pixels (cone_vec-style
+    geometry).
+    """
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n):
+        a = int(rng.choice(_BATCH_SIZES))
+        b = a if rng.uniform() < 0.6 else int(rng.choice(_BATCH_SIZES))
+        shape = tuple(max(8, int(round(s * scale))) for s in (a, a, b))
+        n_angles = int(rng.integers(90, 361))
+        sod = float(rng.uniform(300.0, 800.0))
+        mag = float(rng.uniform(1.2, 2.5))
+        odd = sod * (mag - 1.0)
+        # detector: the magnified volume (its xy diagonal across, z up)
+        # on 96-384 pixels
+        diag = float(np.hypot(shape[0], shape[1]))
+        cols = int(np.clip(8 * round(1.5 * diag / 8), 96 * scale, 384 * scale))
+        rows = int(np.clip(8 * round(1.5 * shape[2] / 8), 96 * scale,
+                           384 * scale))
+        cols, rows = max(cols, 8), max(rows, 8)
+        # the near side of the volume is magnified more: leave 15 % margin
+        pixel = 1.15 * mag * max(diag / cols, shape[2] / rows)
+        pgs = circular_geometry(n_angles, sod, odd, rows, cols, pixel,
+                                tilt_deg=2.0, drift_px=5.0,
+                                seed=int(rng.integers(1 << 31)))
+        vg = resolve_volume_geometry(shape=shape, voxel_size=1.0)
+        out.append((pgs, vg))
+    return out

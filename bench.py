@@ -115,6 +115,131 @@ def cpu_baseline(pgs, vg, vol):
     }
 
 
+def _event_ms(fn, stream, reps=1):
+    """Average milliseconds of ``fn`` between HIP events on ``stream``."""
+    import torch
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    a.record(stream)
+    for _ in range(reps):
+        fn()
+    b.record(stream)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def batched_sirt(n_problems=64, iters=30, seed=0):
+    """SIRT on 64 uneven problems: the whole batch per iteration in two
+    persistent launches, the loop captured in a HIP graph, against the
+    per-problem path (``algo.sirt``'s per-iteration launches, problem after
+    problem).  Setup (textures, weights, tuning, capture) is not timed."""
+    import numpy as np
+    import torch
+
+    from kernelkit import algo, workloads
+    from kernelkit.batch import ConeBatch
+
+    dev = torch.device("cuda", 0)
+    problems = workloads.batch_problems(n_problems, seed=seed)
+    B = ConeBatch(problems)
+    vols = [torch.from_numpy(workloads.ellipsoid_phantom(
+        vg.shape, n=10, seed=seed + k)).to(dev)
+        for k, (_, vg) in enumerate(problems)]
+    ys = B.fp(vols, tune=False)
+    torch.cuda.synchronize()
+    updates = sum(int(np.prod(vg.shape)) * len(pg) for pg, vg in problems)
+    stream = torch.cuda.current_stream(dev)
+    st = B.sirt_state(ys)
+    st.configure(tune=True)
+    st.capture(iters)
+    torch.cuda.synchronize()
+    t_graph = _event_ms(lambda: st._graph[1].replay(), stream)
+    st.reset()
+    t_eager = _event_ms(lambda: [st.iterate() for _ in range(iters)], stream)
+    plain = [algo._SirtState(y, pg, vg, None, None, None)
+             for (pg, vg), y in zip(problems, ys)]
+    torch.cuda.synchronize()
+
+    def loop():
+        for _ in range(iters):
+            for s in plain:
+                s.iterate()
+    t_plain = _event_ms(loop, stream)
+    diff = max(float((a - b).abs().max() / b.abs().max().clamp_min(1e-30))
+               for a, b in zip(st.x, [s.x for s in plain]))
+    del plain, st
+    return {
+        "workload": (f"{n_problems} problems, 64^3-256^3 (cubic and "
+                     "anisotropic), 90-360 angles, 96^2-384^2 detectors, "
+                     f"tilted/shifted cone_vec geometry; {iters} SIRT "
+                     "iterations"),
+        "updates_per_iteration": 2 * updates,
+        "batched_graph_ms": round(t_graph, 2),
+        "batched_eager_ms": round(t_eager, 2),
+        "per_problem_ms": round(t_plain, 2),
+        "speedup_vs_per_problem": round(t_plain / t_graph, 3),
+        "batched_gups": round(2 * updates * iters / (t_graph * 1e-3) / 1e9,
+                              1),
+        "max_rel_diff_vs_per_problem": diff,
+        "tiles": {"fp": st_cfg_repr(B, "fp"), "bp": st_cfg_repr(B, "bp")},
+    }
+
+
+def st_cfg_repr(B, which):
+    cfg = [v for k, v in B.tuner._cache.items() if "sirt" in k]
+    if not cfg:
+        return None
+    c = cfg[0][0 if which == "fp" else 1]
+    return {"tile": c.tile, "grid": c.grid}
+
+
+def autograd_diff():
+    """Forward + backward of ||A x - y||^2 through the exact transpose
+    against two plain calls (forward projection + backprojection) at 256^3,
+    360 angles, 384^2 detector."""
+    import numpy as np
+    import torch
+
+    import kernelkit as kk
+    from kernelkit import workloads
+    from kernelkit.torch_support import XrayForwardProjection
+
+    dev = torch.device("cuda", 0)
+    pgs, vg = workloads.config_geometry("diff")
+    x = torch.from_numpy(workloads.ellipsoid_phantom(vg.shape, n=10,
+                                                     seed=0)).to(dev)
+    A = kk.XrayTransform(pgs, vg)
+    y = A(x)
+    v = torch.empty_like(x)
+    fpm = XrayForwardProjection(pgs, vg, matched=True)
+    xx = x[None, None].clone().requires_grad_(True)
+    yy = (y * 1.01)[None, None]
+    stream = torch.cuda.current_stream(dev)
+
+    def plain():
+        A(x, out=y)
+        A.T(y, out=v)
+
+    def fwd_bwd():
+        xx.grad = None
+        ((fpm(xx) - yy) ** 2).sum().backward()
+
+    plain(), fwd_bwd()
+    t_plain = _event_ms(plain, stream, reps=3)
+    t_ag = _event_ms(fwd_bwd, stream, reps=3)
+    t_t = _event_ms(lambda: A.matched_T(y, out=v), stream, reps=3)
+    updates = int(np.prod(vg.shape)) * len(pgs)
+    return {
+        "workload": "256^3 volume, 360 angles, 384^2 detector",
+        "plain_fp_plus_bp_ms": round(t_plain, 3),
+        "autograd_fwd_bwd_ms": round(t_ag, 3),
+        "ratio_vs_plain": round(t_ag / t_plain, 3),
+        "transpose_ms": round(t_t, 3),
+        "transpose_gups": round(updates / (t_t * 1e-3) / 1e9, 1),
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,6 +248,9 @@ def main(argv=None):
     ap.add_argument("--config", default="full",
                     help="workload: full (default), small, diff")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--quick", action="store_true",
+                    help="only the FP/BP step (no batched SIRT or autograd "
+                         "comparison)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     metavar="MIB",
                     help="after the timed steps, run one plain device copy "
@@ -269,6 +397,11 @@ def main(argv=None):
         },
         "cpu_baseline": None,
     }
+    if not args.quick and args.config == "full":
+        _log("batched SIRT vs per-problem ...")
+        out["batched"] = batched_sirt()
+        _log("autograd (exact transpose) vs plain ...")
+        out["autograd"] = autograd_diff()
     if not args.no_cpu_baseline:
         _log("cpu baseline (FP64 oracle, 1 core) ...")
         out["cpu_baseline"] = cpu_baseline(pgs, vg, vol)

@@ -124,6 +124,67 @@ int kk_cone_bp(const kk_plan* plan, const float* proj_tex, int32_t n_angles,
                const int32_t vol_shape[3], const int64_t vol_strides[3],
                float output_scale, int32_t additive, void* stream);
 
+/* Exact transpose of kk_cone_fp (matched adjoint, for gradients): every
+ * (ray, slice) sample of the forward projection adds proj * scale *
+ * bilinear weight to its in-volume taps, with the forward kernel's float32
+ * arithmetic, so <A x, y> = <x, A^T y> up to rounding.  No counterpart in
+ * the reference, whose autograd backward uses the voxel-driven
+ * backprojector (kernelkit/torch_support.py:29-33); it replaces that call
+ * where an exact gradient is wanted.  proj: (angle, row, col) strides; vol:
+ * world strides, shape equal to the parameters' volume shape.  additive !=
+ * 0: vol += A^T proj, else vol = A^T proj. */
+int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
+                         const int64_t proj_strides[3], int32_t rows,
+                         int32_t cols, float* vol, const int32_t vol_shape[3],
+                         const int64_t vol_strides[3], int32_t additive,
+                         void* stream);
+
+/* ---- batches of independent problems (one launch per operator) ------ */
+
+/* No counterpart in the reference, which runs one geometry per projector
+ * (kernelkit/projector.py:160-183) and loops over samples in Python
+ * (kernelkit/torch_support.py:19-20).  A batch records problems with the
+ * arguments of kk_cone_fp, kk_cone_bp, kk_sirt_fp_residual and
+ * kk_sirt_bp_update (pointers are kept, not copied: they must stay valid),
+ * kk_batch_configure builds device descriptors for one tile shape
+ * (fp_rows_per_thread 1/2/4 or 0 = automatic; bp_zrun 4/8/16), and
+ * kk_batch_fp / kk_batch_bp run all forward / backward problems in one
+ * persistent launch of grid_blocks workgroups (0 = one per tile).  Launches
+ * allocate nothing and are graph-capturable; a SIRT iteration of the batch
+ * is kk_batch_fp then kk_batch_bp.  Adding a problem drops the descriptors
+ * (configure again). */
+typedef struct kk_batch_s kk_batch;
+int kk_batch_create(kk_batch** out);
+int kk_batch_destroy(kk_batch* batch);
+/* Number of forward (which = 0) or backward (which = 1) problems. */
+int32_t kk_batch_size(const kk_batch* batch, int32_t which);
+int kk_batch_add_fp(kk_batch* batch, const kk_plan* plan,
+                    const float* vol_tex, float* proj,
+                    const int64_t proj_strides[3], int32_t rows, int32_t cols,
+                    int32_t additive);
+int kk_batch_add_bp(kk_batch* batch, const kk_plan* plan,
+                    const float* proj_tex, int32_t n_angles, int32_t rows,
+                    int32_t cols, float* vol, const int32_t vol_shape[3],
+                    const int64_t vol_strides[3], float output_scale,
+                    int32_t additive);
+int kk_batch_add_sirt_fp(kk_batch* batch, const kk_plan* plan,
+                         const float* x_tex, const float* y,
+                         const int64_t y_strides[3], const float* R,
+                         const int64_t R_strides[3], float* res_tex,
+                         int32_t n_angles, int32_t rows, int32_t cols);
+int kk_batch_add_sirt_bp(kk_batch* batch, const kk_plan* plan,
+                         const float* res_tex, int32_t n_angles, int32_t rows,
+                         int32_t cols, float* x, const int32_t vol_shape[3],
+                         const int64_t x_strides[3], const float* C,
+                         const int64_t C_strides[3], const float* mask,
+                         const int64_t mask_strides[3], float lo, float hi,
+                         int32_t clip_lo, int32_t clip_hi, float* x_tex,
+                         int32_t x_tex_mask, float output_scale);
+int kk_batch_configure(kk_batch* batch, int32_t fp_rows_per_thread,
+                       int32_t bp_zrun);
+int kk_batch_fp(const kk_batch* batch, int32_t grid_blocks, void* stream);
+int kk_batch_bp(const kk_batch* batch, int32_t grid_blocks, void* stream);
+
 /* ---- host-only introspection (no device calls) ---------------------- */
 
 /* The per-angle device records that kk_set_params_fp / kk_set_params_bp

@@ -1,0 +1,126 @@
+"""Batched problems (one launch per operator) against the per-problem
+path, the oracle and a HIP-graph replay of SIRT."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import assert_parity
+
+pytestmark = pytest.mark.gpu
+
+kk = pytest.importorskip("kernelkit")
+from kernelkit import workloads  # noqa: E402
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def _problems(n=5, seed=1):
+    return workloads.batch_problems(n, seed=seed, scale=0.25)
+
+
+def _dev(a):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).cuda()
+
+
+def _host(t):
+    return t.detach().cpu().numpy()
+
+
+def _volumes(problems, seed=0):
+    return [workloads.ellipsoid_phantom(vg.shape, n=4, seed=seed + k)
+            for k, (_, vg) in enumerate(problems)]
+
+
+def test_batch_problem_sizes_are_uneven(device):
+    P = workloads.batch_problems(64)
+    shapes = {vg.shape for _, vg in P}
+    angles = {len(pg) for pg, _ in P}
+    dets = {(pg[0].detector.rows, pg[0].detector.cols) for pg, _ in P}
+    assert len(shapes) > 10 and len(angles) > 30 and len(dets) > 5
+    assert all(90 <= len(pg) <= 360 for pg, _ in P)
+
+
+@pytest.mark.parametrize("tune", [False, True])
+def test_batch_fp_bp_match_single(tune, device):
+    from kernelkit.autotune import BatchTuner
+    from kernelkit.batch import ConeBatch
+    P = _problems()
+    B = ConeBatch(P, tuner=BatchTuner(reps=1))
+    vols = [_dev(v) for v in _volumes(P)]
+    ys = B.fp(vols, tune=tune)
+    for (pg, vg), v, y in zip(P, vols, ys):
+        ref = kk.XrayTransform(pg, vg)(v)
+        np.testing.assert_allclose(_host(y), _host(ref), rtol=1e-6,
+                                   atol=1e-6 * float(ref.abs().max()))
+    xs = B.bp(ys, tune=tune)
+    for (pg, vg), y, x in zip(P, ys, xs):
+        ref = kk.XrayTransform(pg, vg).T(y)
+        np.testing.assert_allclose(_host(x), _host(ref), rtol=1e-6,
+                                   atol=1e-6 * float(ref.abs().max()))
+
+
+def test_batch_matches_oracle(device):
+    from kernelkit.batch import ConeBatch
+    # a few angles each keep the oracle fast
+    P = [(pg[::max(1, len(pg) // 16)], vg) for pg, vg in _problems(3, seed=4)]
+    vols = _volumes(P, seed=3)
+    B = ConeBatch(P)
+    ys = B.fp([_dev(v) for v in vols], tune=False)
+    for (pg, vg), v, y in zip(P, vols, ys):
+        assert_parity(_host(y), oracle.cone_fp(v, pg, vg), label="batch fp")
+    xs = B.bp(ys, tune=False)
+    for (pg, vg), y, x in zip(P, ys, xs):
+        assert_parity(_host(x), oracle.cone_bp(_host(y), pg, vg),
+                      label="batch bp")
+
+
+def test_batch_sirt_matches_plain_and_graph_replays(device):
+    torch = _torch()
+    from kernelkit.batch import ConeBatch
+    P = _problems(4, seed=7)
+    vols = _volumes(P, seed=5)
+    ys = [kk.XrayTransform(pg, vg)(_dev(v)) for (pg, vg), v in zip(P, vols)]
+    refs = [kk.sirt(y, pg, vg, iters=6, verbose=False, min_constraint=0.0,
+                    return_gpu=True) for (pg, vg), y in zip(P, ys)]
+    B = ConeBatch(P)
+    st = B.sirt_state(ys, min_constraint=0.0)
+    st.configure(tune=True)
+    st.run(3, graph=False)
+    st.run(3, graph=False)
+    for x, r in zip(st.x, refs):
+        np.testing.assert_allclose(_host(x), _host(r), rtol=2e-5,
+                                   atol=2e-5 * float(r.abs().max()))
+    # graph: capture 3 iterations, replay twice with eager work between
+    st.reset()
+    st.capture(3)
+    st._graph[1].replay()
+    torch.cuda.synchronize()
+    eager = [r.clone() for r in refs]  # keep references alive
+    _ = torch.ones(10, device="cuda").sum().item()
+    st._graph[1].replay()
+    torch.cuda.synchronize()
+    for x, r in zip(st.x, eager):
+        np.testing.assert_allclose(_host(x), _host(r), rtol=2e-5,
+                                   atol=2e-5 * float(r.abs().max()))
+    # a third replay continues iterating: matches 9 plain iterations
+    st._graph[1].replay()
+    torch.cuda.synchronize()
+    for (pg, vg), y, x in zip(P, ys, st.x):
+        r = kk.sirt(y, pg, vg, iters=9, verbose=False, min_constraint=0.0,
+                    return_gpu=True)
+        np.testing.assert_allclose(_host(x), _host(r), rtol=2e-5,
+                                   atol=2e-5 * float(r.abs().max()))
+
+
+def test_batch_shape_errors(device):
+    from kernelkit.batch import ConeBatch
+    P = _problems(2)
+    B = ConeBatch(P)
+    with pytest.raises(ValueError):
+        B.fp([_dev(np.zeros((3, 3, 3)))] * 2)
+    with pytest.raises(ValueError):
+        B.fp([_dev(np.zeros(P[0][1].shape))])

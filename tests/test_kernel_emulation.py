@@ -178,3 +178,133 @@ def test_full_config_fp_table_precision():
                 emu = a1f * np.float32(i) + (a1f * np.float32(P["sig"]) +
                                              np.float32(P["taub"]))
                 assert abs(float(emu) - exact) < 1e-3
+
+
+_TS, _TB, _TC = 8, 32, 32   # cone_fp_t_kernel tile (slices, b, c)
+
+
+def _project_corner(P, xa, xb, xc):
+    """cone_fp_t_kernel's project(): detector (col, row) of the ray through
+    the source-relative point X, float32; ok iff finite and in front."""
+    g = {k: f32(P[k]) for k in ("qa", "qb", "qc", "ua", "ub", "uc",
+                                "va", "vb", "vc")}
+    xa, xb, xc = f32(xa), f32(xb), f32(xc)
+    with np.errstate(all="ignore"):
+        vx = (g["vb"] * xc - g["vc"] * xb, g["vc"] * xa - g["va"] * xc,
+              g["va"] * xb - g["vb"] * xa)
+        qx = (g["qb"] * xc - g["qc"] * xb, g["qc"] * xa - g["qa"] * xc,
+              g["qa"] * xb - g["qb"] * xa)
+        vq = (g["vb"] * g["qc"] - g["vc"] * g["qb"],
+              g["vc"] * g["qa"] - g["va"] * g["qc"],
+              g["va"] * g["qb"] - g["vb"] * g["qa"])
+        den = g["ua"] * vx[0] + g["ub"] * vx[1] + g["uc"] * vx[2]
+        inv = f32(1) / den
+        col = -(g["qa"] * vx[0] + g["qb"] * vx[1] + g["qc"] * vx[2]) * inv
+        row = -(g["ua"] * qx[0] + g["ub"] * qx[1] + g["uc"] * qx[2]) * inv
+        lam = (g["ua"] * vq[0] + g["ub"] * vq[1] + g["uc"] * vq[2]) * inv
+    return col, row, bool(np.isfinite(col) and np.isfinite(row) and lam > 0)
+
+
+def emulate_fp_t(proj, table, shape, use_box=True, tile=(_TS, _TB, _TC)):
+    """cone_fp_t_kernel: per frame, per tile, per angle, the rays of the
+    projected pixel rectangle add their in-tile taps.  Coordinates and
+    weights in float32 as the kernel; sums in float64 (the kernel's LDS
+    accumulation order is not defined).  use_box=False walks every ray of
+    the detector instead (to check that the rectangle loses nothing)."""
+    n, rows, cols = proj.shape
+    lo = _native.HALO_LO
+    TS, TB, TC = tile
+    out = np.zeros(shape)
+    for a, (_, b, c) in _FRAMES.items():
+        na, nb, nc = shape[a], shape[b], shape[c]
+        acc = np.zeros((na, nb, nc))
+        angles = [k for k in range(n) if int(table[k]["axis"]) == a]
+        for i0 in range(0, na, TS):
+            ns = min(TS, na - i0)
+            for j0 in range(0, nb, TB):
+                nbt = min(TB, nb - j0)
+                for k0 in range(0, nc, TC):
+                    nct = min(TC, nc - k0)
+                    for k in angles:
+                        P = table[k]
+                        c_lo, c_hi, r_lo, r_hi = 0, cols - 1, 0, rows - 1
+                        if use_box:
+                            pts = [_project_corner(
+                                P, f32(ia) + f32(P["sig"]),
+                                f32(jb) - f32(P["taub"]),
+                                f32(jc) - f32(P["tauc"]))
+                                for ia in (i0, i0 + ns - 1)
+                                for jb in (j0, j0 + nbt + 1)
+                                for jc in (k0, k0 + nct + 1)]
+                            if all(p[2] for p in pts):
+                                cs = [p[0] for p in pts]
+                                rs = [p[1] for p in pts]
+                                c_lo = max(0, int(np.floor(np.clip(
+                                    min(cs), -4, 1e8))) - 1)
+                                c_hi = min(cols - 1, int(np.ceil(np.clip(
+                                    max(cs), -4, 1e8))) + 1)
+                                r_lo = max(0, int(np.floor(np.clip(
+                                    min(rs), -4, 1e8))) - 1)
+                                r_hi = min(rows - 1, int(np.ceil(np.clip(
+                                    max(rs), -4, 1e8))) + 1)
+                        if c_lo > c_hi or r_lo > r_hi:
+                            continue
+                        R, C = np.meshgrid(np.arange(r_lo, r_hi + 1),
+                                           np.arange(c_lo, c_hi + 1),
+                                           indexing="ij")
+                        R, C = R.ravel(), C.ravel()
+                        fr, fcl = R.astype(f32), C.astype(f32)
+                        Da = fr * f32(P["va"]) + (fcl * f32(P["ua"]) +
+                                                  f32(P["qa"]))
+                        Db = fr * f32(P["vb"]) + (fcl * f32(P["ub"]) +
+                                                  f32(P["qb"]))
+                        Dc = fr * f32(P["vc"]) + (fcl * f32(P["uc"]) +
+                                                  f32(P["qc"]))
+                        with np.errstate(all="ignore"):
+                            a1 = (Db / Da).astype(f32)
+                            a2 = (Dc / Da).astype(f32)
+                        b1 = (a1 * f32(P["sig"]) + f32(P["taub"])).astype(f32)
+                        b2 = (a2 * f32(P["sig"]) + f32(P["tauc"])).astype(f32)
+                        w = (np.sqrt(a1 * a1 * f32(P["sc1"]) + a2 * a2 *
+                                     f32(P["sc2"]) + f32(1)) *
+                             f32(P["oscale"])).astype(f32)
+                        val = (proj[k, R, C].astype(f32) * w).astype(
+                            np.float64)
+                        for s in range(ns):
+                            i = i0 + s
+                            xb = np.clip(a1 * f32(i) + b1, 0, nb + 1)
+                            xc = np.clip(a2 * f32(i) + b2, 0, nc + 1)
+                            ib = np.floor(xb).astype(np.int64) - lo
+                            ic = np.floor(xc).astype(np.int64) - lo
+                            fb = (xb - np.floor(xb)).astype(np.float64)
+                            fc = (xc - np.floor(xc)).astype(np.float64)
+                            for db, wb in ((0, 1 - fb), (1, fb)):
+                                for dc, wc in ((0, 1 - fc), (1, fc)):
+                                    jj, kk_ = ib + db, ic + dc
+                                    m = ((jj >= j0) & (jj < j0 + nbt) &
+                                         (kk_ >= k0) & (kk_ < k0 + nct))
+                                    np.add.at(acc[i], (jj[m], kk_[m]),
+                                              (val * wb * wc)[m])
+        out += np.transpose(acc, np.argsort((a, b, c)))
+    return out
+
+
+@pytest.mark.parametrize("pert", [{}, {"tilt_deg": 2.0, "drift_px": 4.0}])
+def test_fp_transpose_emulation(pert):
+    """The tiled transpose equals the oracle's exact transpose, and its
+    per-tile pixel rectangles lose no ray."""
+    pgs, vg, _ = _case(n_angles=5, shape=(20, 40, 36), rows=30, cols=44,
+                       **pert)
+    pgs = list(pgs) + [kk.rotate(pgs[0], pitch=np.pi / 2)]   # + a z frame
+    table = _fp_table(pgs, vg)
+    assert set(table["axis"].tolist()) >= {2}
+    y = np.random.default_rng(3).uniform(size=(len(pgs), 30, 44))
+    got = emulate_fp_t(y, table, vg.shape)
+    assert_parity(got, oracle.cone_fp_transpose(y, pgs, vg),
+                  label=f"fp_t-emu {pert}")
+    # small tiles make the pixel rectangles a small part of the detector:
+    # walking every ray instead changes nothing
+    small = emulate_fp_t(y, table, vg.shape, tile=(3, 5, 4))
+    full = emulate_fp_t(y, table, vg.shape, use_box=False, tile=(3, 5, 4))
+    np.testing.assert_allclose(small, full, rtol=1e-12, atol=1e-12)
+    np.testing.assert_allclose(small, got, rtol=1e-12, atol=1e-12)
