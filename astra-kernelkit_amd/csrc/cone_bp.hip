@@ -62,6 +62,27 @@ __device__ __forceinline__ float bilerp(kk_float4u t, float fu, float fv) {
   return fmaf(fu, l1 - l0, l0);
 }
 
+// 1/den from a float32 estimate and one float64 Newton step (relative
+// error ~1e-14).
+__device__ __forceinline__ double refine_rcp(double den, float rf) {
+  const double r = static_cast<double>(rf);
+  return fma(r, fma(-den, r, 1.0), r);
+}
+
+// Coordinate base + off (base an integer, off a small float) clamped into
+// [0, hi] (the zero halo), split into element index and weight: the same
+// result as clamping the float64 coordinate, then floor and fraction.
+__device__ __forceinline__ void split_clamped(float off, int32_t base,
+                                              int32_t hi, int32_t& idx,
+                                              float& frac) {
+  const float o = kk_clamp(off, -1e7f, 1e7f);
+  const float fl = floorf(o);
+  const int32_t i = base + static_cast<int32_t>(fl);
+  const bool out = i < 0 || i >= hi;
+  frac = out ? 0.f : o - fl;
+  idx = min(max(i, 0), hi);
+}
+
 // One workgroup's tile (16 x 16 columns of K voxels) of logical block id
 // `bid`.
 template <int K>
@@ -76,11 +97,15 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
   const int32_t iz0 = tz * K;
   if (ix >= A.nx || iy >= A.ny) return;
 
-  const float X = static_cast<float>(ix) - 0.5f * A.nx + 0.5f;
-  const float Y = static_cast<float>(iy) - 0.5f * A.ny + 0.5f;
-  const float Z0 = static_cast<float>(iz0) - 0.5f * A.nz + 0.5f;
-  const float rmax = static_cast<float>(A.rows + 1);
-  const float cmax = static_cast<float>(A.cols + 1);
+  // voxel centres; the geometry is evaluated in float64 per (x, y, angle):
+  // detector coordinates reach ~cols + 1 (up to ~1.5e3), where float32
+  // would round them by ~1e-4 pixel.  Per voxel only small offsets from an
+  // integer base are float32.
+  const double X = static_cast<double>(ix) - 0.5 * A.nx + 0.5;
+  const double Y = static_cast<double>(iy) - 0.5 * A.ny + 0.5;
+  const double Z0 = static_cast<double>(iz0) - 0.5 * A.nz + 0.5;
+  const int32_t rmax = A.rows + 1;
+  const int32_t cmax = A.cols + 1;
   const uint32_t trb = A.trb;
 
   float acc[K];
@@ -90,29 +115,36 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
   const char* img = A.tex;
   for (int32_t a = 0; a < A.n_angles; ++a, img += A.tab_b) {
     const kk::BpAngle& P = tab[a];
-    const float den = fmaf(Y, P.dy, fmaf(X, P.dx, P.dw));
-    const float nu = fmaf(Y, P.nuy, fmaf(X, P.nux, P.nuw));
-    const float nv = fmaf(Y, P.nvy, fmaf(X, P.nvx, P.nvw));
+    const double den = fma(Y, P.dy, fma(X, P.dx, P.dw));
+    const double nu = fma(Y, P.nuy, fma(X, P.nux, P.nuw));
+    const double nv = fma(Y, P.nvy, fma(X, P.nvx, P.nvw));
     if (P.fast) {
-      const float r = __builtin_amdgcn_rcpf(den);
-      const float un = nu * r;
-      const float v0 = fmaf(Z0, P.nvz, nv) * r;
-      const float gv = P.nvz * r;
+      // u and 1/den do not depend on z; v is affine in z
+      const float rf = __builtin_amdgcn_rcpf(static_cast<float>(den));
+      const double r = refine_rcp(den, rf);
+      const double u = nu * r;
+      const double v0 = fma(Z0, P.nvz, nv) * r;
+      const float gv = static_cast<float>(P.nvz * r);
       // a voxel in the plane of the source parallel to the detector
       // (den = 0) has no projection: skip it
-      if (!(isfinite(un) && isfinite(v0) && isfinite(gv))) continue;
-      const float w = r * r;
-      const float u = kk_clamp(un, 0.f, cmax);
-      const float fu = __builtin_amdgcn_fractf(u);
-      const uint32_t offu = static_cast<uint32_t>(u) << 3;
+      if (!(isfinite(u) && isfinite(v0) && isfinite(gv))) continue;
+      const float w = rf * rf;
+      const double uc = fmin(fmax(u, 0.0), static_cast<double>(cmax));
+      const double ufl = floor(uc);
+      const float fu = static_cast<float>(uc - ufl);
+      const uint32_t offu = static_cast<uint32_t>(ufl) << 3;
+      const double vc0 = fmin(fmax(v0, -1e6), 1e6);
+      const double vfl = floor(vc0);
+      const int32_t ivb = static_cast<int32_t>(vfl);
+      const float fv0 = static_cast<float>(vc0 - vfl);
       uint32_t off[K];
       float fv[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const float v = kk_clamp(fmaf(static_cast<float>(k), gv, v0), 0.f,
-                                 rmax);
-        fv[k] = __builtin_amdgcn_fractf(v);
-        off[k] = __umul24(static_cast<uint32_t>(v), trb) + offu;
+        int32_t iv;
+        split_clamped(fmaf(static_cast<float>(k), gv, fv0), ivb, rmax, iv,
+                      fv[k]);
+        off[k] = __umul24(static_cast<uint32_t>(iv), trb) + offu;
       }
       kk_float4u tp[K];
 #pragma unroll
@@ -121,21 +153,41 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
       for (int k = 0; k < K; ++k)
         acc[k] = fmaf(w, bilerp(tp[k], fu, fv[k]), acc[k]);
     } else {
+      // integer bases at the run's first voxel; per voxel,
+      // u(k) - ub = (e0u + k e1u) / den(k) with float32 numerators
+      const double den0 = fma(Z0, P.dz, den);
+      const double nu0 = fma(Z0, P.nuz, nu);
+      const double nv0 = fma(Z0, P.nvz, nv);
+      const double r0 = refine_rcp(
+          den0, __builtin_amdgcn_rcpf(static_cast<float>(den0)));
+      double ub = 0.0, vb = 0.0;
+      const double u0 = nu0 * r0, v0 = nv0 * r0;
+      if (isfinite(u0) && isfinite(v0)) {
+        ub = floor(fmin(fmax(u0, -1e6), 1e6));
+        vb = floor(fmin(fmax(v0, -1e6), 1e6));
+      }
+      const float e0u = static_cast<float>(fma(-ub, den0, nu0));
+      const float e1u = static_cast<float>(fma(-ub, P.dz, P.nuz));
+      const float e0v = static_cast<float>(fma(-vb, den0, nv0));
+      const float e1v = static_cast<float>(fma(-vb, P.dz, P.nvz));
+      const float d0f = static_cast<float>(den0);
+      const float dzf = static_cast<float>(P.dz);
+      const int32_t iub = static_cast<int32_t>(ub);
+      const int32_t ivb = static_cast<int32_t>(vb);
       uint32_t off[K];
       float fu[K], fv[K], w[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
-        const float Z = Z0 + static_cast<float>(k);
-        const float r = __builtin_amdgcn_rcpf(fmaf(Z, P.dz, den));
-        const float un = fmaf(Z, P.nuz, nu) * r;
-        const float vn = fmaf(Z, P.nvz, nv) * r;
-        const bool ok = isfinite(un) && isfinite(vn);
-        const float u = kk_clamp(ok ? un : 0.f, 0.f, cmax);
-        const float v = kk_clamp(ok ? vn : 0.f, 0.f, rmax);
-        fu[k] = __builtin_amdgcn_fractf(u);
-        fv[k] = __builtin_amdgcn_fractf(v);
-        off[k] = __umul24(static_cast<uint32_t>(v), trb) +
-                 (static_cast<uint32_t>(u) << 3);
+        const float fk = static_cast<float>(k);
+        const float r = __builtin_amdgcn_rcpf(fmaf(fk, dzf, d0f));
+        const float du = fmaf(fk, e1u, e0u) * r;
+        const float dv = fmaf(fk, e1v, e0v) * r;
+        const bool ok = isfinite(du) && isfinite(dv);
+        int32_t iu, iv;
+        split_clamped(ok ? du : 0.f, ok ? iub : 0, cmax, iu, fu[k]);
+        split_clamped(ok ? dv : 0.f, ok ? ivb : 0, rmax, iv, fv[k]);
+        off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
+                 (static_cast<uint32_t>(iu) << 3);
         w[k] = ok ? r * r : 0.f;
       }
       kk_float4u tp[K];

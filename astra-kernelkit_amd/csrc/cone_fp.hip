@@ -96,36 +96,37 @@ __device__ __forceinline__ float bilerp(kk_float4u t, float fb, float fc) {
 template <int R, bool COLC>
 __device__ __forceinline__ void trace(const kk::FpAngle& P,
                                       const char* __restrict__ frame,
-                                      const float (&a1)[R],
-                                      const float (&b1)[R],
-                                      const float (&a2)[R],
-                                      const float (&b2)[R], int32_t i0,
+                                      const double (&a1)[R],
+                                      const double (&b1)[R],
+                                      const double (&a2)[R],
+                                      const double (&b2)[R], int32_t i0,
                                       int32_t i1, float (&acc)[R]) {
-  const float nb1 = static_cast<float>(P.nb + 1);
-  const float nc1 = static_cast<float>(P.nc + 1);
+  const double nb1 = static_cast<double>(P.nb + 1);
+  const double nc1 = static_cast<double>(P.nc + 1);
   const uint32_t sab = P.sab, sbb = P.sbb;
   const char* slice = frame + static_cast<size_t>(i0) * sab;
   for (int32_t i = i0; i <= i1; ++i, slice += sab) {
-    const float fi = static_cast<float>(i);
+    const double fi = static_cast<double>(i);
     float fbs = 0.f;
     uint32_t offb = 0;
     if constexpr (COLC) {
-      const float xb = kk_clamp(fmaf(a1[0], fi, b1[0]), 0.f, nb1);
-      fbs = __builtin_amdgcn_fractf(xb);
-      offb = __umul24(static_cast<uint32_t>(xb), sbb);
+      uint32_t ib;
+      kk_fp_coord(a1[0], b1[0], fi, nb1, ib, fbs);
+      offb = __umul24(ib, sbb);
     }
 #pragma unroll
     for (int k = 0; k < R; ++k) {
       float fb = fbs;
       uint32_t ob = offb;
       if constexpr (!COLC) {
-        const float xb = kk_clamp(fmaf(a1[k], fi, b1[k]), 0.f, nb1);
-        fb = __builtin_amdgcn_fractf(xb);
-        ob = __umul24(static_cast<uint32_t>(xb), sbb);
+        uint32_t ib;
+        kk_fp_coord(a1[k], b1[k], fi, nb1, ib, fb);
+        ob = __umul24(ib, sbb);
       }
-      const float xc = kk_clamp(fmaf(a2[k], fi, b2[k]), 0.f, nc1);
-      const float fc = __builtin_amdgcn_fractf(xc);
-      const uint32_t off = ob + (static_cast<uint32_t>(xc) << 3);
+      uint32_t ic;
+      float fc;
+      kk_fp_coord(a2[k], b2[k], fi, nc1, ic, fc);
+      const uint32_t off = ob + (ic << 3);
       acc[k] += bilerp(kk_ld4(slice, off), fb, fc);
     }
   }
@@ -148,36 +149,29 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
   const float nb1 = static_cast<float>(P.nb + 1);
   const float nc1 = static_cast<float>(P.nc + 1);
   const float last = static_cast<float>(P.na - 1);
-  const float fcl = static_cast<float>(col);
 
-  float a1[R], b1[R], a2[R], b2[R], wscale[R];
+  double a1[R], b1[R], a2[R], b2[R];
+  float wscale[R];
   int32_t ilo = 0x7fffffff, ihi = -1;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int32_t row = row0 + kWave * k;
-    const float fr = static_cast<float>(row);
-    const float Da = fmaf(fr, P.va, fmaf(fcl, P.ua, P.qa));
-    const float Db = fmaf(fr, P.vb, fmaf(fcl, P.ub, P.qb));
-    const float Dc = fmaf(fr, P.vc, fmaf(fcl, P.uc, P.qc));
-    a1[k] = Db / Da;
-    a2[k] = Dc / Da;
-    b1[k] = fmaf(a1[k], P.sig, P.taub);
-    b2[k] = fmaf(a2[k], P.sig, P.tauc);
     // rays outside the detector, or parallel to the slices, contribute
     // nothing: park them in the halo
-    const bool ok = row < A.rows && col < A.cols && isfinite(a1[k]) &&
-                    isfinite(a2[k]) && isfinite(b1[k]) && isfinite(b2[k]);
+    const bool ok = row < A.rows && col < A.cols &&
+                    kk_fp_ray(P, row, col, a1[k], b1[k], a2[k], b2[k]);
     wscale[k] = 0.f;
     if (!ok) {
-      a1[k] = a2[k] = 0.f;
-      b1[k] = b2[k] = -1.f;
+      a1[k] = a2[k] = 0.0;
+      b1[k] = b2[k] = -1.0;
       continue;
     }
-    wscale[k] = sqrtf(fmaf(a1[k] * a1[k], P.sc1,
-                           fmaf(a2[k] * a2[k], P.sc2, 1.f))) * P.oscale;
+    wscale[k] = kk_fp_weight(P, a1[k], a2[k]);
     float lo = 0.f, hi = last;
-    clip_range(a1[k], b1[k], nb1, last, lo, hi);
-    clip_range(a2[k], b2[k], nc1, last, lo, hi);
+    clip_range(static_cast<float>(a1[k]), static_cast<float>(b1[k]), nb1,
+               last, lo, hi);
+    clip_range(static_cast<float>(a2[k]), static_cast<float>(b2[k]), nc1,
+               last, lo, hi);
     if (lo <= hi + 2.f) {
       ilo = min(ilo, max(0, static_cast<int32_t>(floorf(lo)) - 1));
       ihi = max(ihi, min(P.na - 1, static_cast<int32_t>(ceilf(hi)) + 1));

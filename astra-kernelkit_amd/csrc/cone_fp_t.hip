@@ -4,20 +4,21 @@
 // The reference has no such operator: its autograd backward uses the
 // voxel-driven backprojector (kernelkit/torch_support.py:29-33), which is
 // not the transpose of the forward projector.  Here every (ray, slice)
-// sample of the forward kernel, with the same float32 arithmetic (ray
-// slopes, intercepts, clamping into the zero halo, tap index and weights),
-// adds value * scale * bilinear weight to each of its four taps that lie
-// inside the volume, so <A x, y> = <x, A^T y> up to rounding.
+// sample of the forward kernel, with the same arithmetic (kk_fp_ray,
+// kk_fp_weight, kk_fp_coord: ray slopes, intercepts, clamping into the zero
+// halo, tap index and weights), adds value * scale * bilinear weight to
+// each of its four taps that lie inside the volume, so
+// <A x, y> = <x, A^T y> up to rounding.
 //
 // MI355X design: a scatter with one global float atomic per tap would put
 // ~4 atomics per sample on the L2.  Instead each workgroup owns a tile of
-// the volume in the ray frame of one axis (kTS slices x kTB x kTC voxels,
-// 33 KiB of LDS) and accumulates it in LDS over every angle of that axis:
-// per angle it projects the tile's corners onto the detector, walks the
-// rays of the covering pixel rectangle through the tile's slices and adds
-// their taps with LDS float atomics (ds_add_f32).  The tile is written to
-// HBM once.  Angles of different ray axes run as separate launches over
-// tiles of their own frame, the later ones adding.
+// the volume in the ray frame of one axis (kTS slices x kTB x kTC voxels
+// plus a halo, 37 KiB of LDS) and accumulates it in LDS over every angle
+// of that axis: per angle it projects the tile's corners onto the detector,
+// walks the rays of the covering pixel rectangle through the tile's slices
+// and adds their taps with LDS float atomics (ds_add_f32).  The tile is
+// written to HBM once.  Angles of different ray axes run as separate
+// launches over tiles of their own frame, the later ones adding.
 #include "kk_internal.h"
 
 namespace {
@@ -25,10 +26,13 @@ namespace {
 constexpr int kTS = 8;    // slices (ray axis a) per tile
 constexpr int kTB = 32;   // voxels along b per tile
 constexpr int kTC = 32;   // voxels along c per tile
-// LDS row stride: one float of padding, so the lanes of a wavefront (rays
-// of consecutive detector columns, i.e. consecutive b rows of the tile) hit
-// different LDS banks instead of one bank 32 floats apart
-constexpr int kTCP = kTC + 1;
+// The LDS tile has a one-voxel halo around the b and c extents: taps that
+// fall outside the tile are clamped into it and never written back, so the
+// sample loop has no bounds tests.  Row stride kTCP (odd): the lanes of a
+// wavefront (rays of consecutive detector columns, i.e. consecutive b rows)
+// hit different LDS banks.
+constexpr int kTBH = kTB + 2;
+constexpr int kTCP = kTC + 3;
 constexpr int kThreads = 256;
 
 struct FptArgs {
@@ -47,30 +51,32 @@ struct FptArgs {
 // Detector coordinates (col, row) in pixel-centre units of the ray from
 // the source through the source-relative point X (frame components): the
 // solution of q + col U + row V = lam X; ok iff finite with lam > 0.
-__device__ __forceinline__ void project(const kk::FpAngle& P, float xa,
-                                        float xb, float xc, float& col,
+__device__ __forceinline__ void project(const kk::FpAngle& P, double xa,
+                                        double xb, double xc, float& col,
                                         float& row, bool& ok) {
   // V x X, q x X, V x q
-  const float vx0 = P.vb * xc - P.vc * xb;
-  const float vx1 = P.vc * xa - P.va * xc;
-  const float vx2 = P.va * xb - P.vb * xa;
-  const float qx0 = P.qb * xc - P.qc * xb;
-  const float qx1 = P.qc * xa - P.qa * xc;
-  const float qx2 = P.qa * xb - P.qb * xa;
-  const float vq0 = P.vb * P.qc - P.vc * P.qb;
-  const float vq1 = P.vc * P.qa - P.va * P.qc;
-  const float vq2 = P.va * P.qb - P.vb * P.qa;
-  const float den = P.ua * vx0 + P.ub * vx1 + P.uc * vx2;
-  const float inv = 1.f / den;
-  col = -(P.qa * vx0 + P.qb * vx1 + P.qc * vx2) * inv;
-  row = -(P.ua * qx0 + P.ub * qx1 + P.uc * qx2) * inv;
-  const float lam = (P.ua * vq0 + P.ub * vq1 + P.uc * vq2) * inv;
-  ok = isfinite(col) && isfinite(row) && lam > 0.f;
+  const double vx0 = P.vb * xc - P.vc * xb;
+  const double vx1 = P.vc * xa - P.va * xc;
+  const double vx2 = P.va * xb - P.vb * xa;
+  const double qx0 = P.qb * xc - P.qc * xb;
+  const double qx1 = P.qc * xa - P.qa * xc;
+  const double qx2 = P.qa * xb - P.qb * xa;
+  const double vq0 = P.vb * P.qc - P.vc * P.qb;
+  const double vq1 = P.vc * P.qa - P.va * P.qc;
+  const double vq2 = P.va * P.qb - P.vb * P.qa;
+  const double den = P.ua * vx0 + P.ub * vx1 + P.uc * vx2;
+  const double c = -(P.qa * vx0 + P.qb * vx1 + P.qc * vx2) / den;
+  const double r = -(P.ua * qx0 + P.ub * qx1 + P.uc * qx2) / den;
+  const double lam = (P.ua * vq0 + P.ub * vq1 + P.uc * vq2) / den;
+  ok = isfinite(c) && isfinite(r) && lam > 0.0 && fabs(c) < 1e8 &&
+       fabs(r) < 1e8;
+  col = static_cast<float>(ok ? c : 0.0);
+  row = static_cast<float>(ok ? r : 0.0);
 }
 
 __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
     const kk::FpAngle* __restrict__ tab, FptArgs A) {
-  __shared__ float acc[kTS * kTB * kTCP];
+  __shared__ float acc[kTS * kTBH * kTCP];
   int64_t bid = blockIdx.x;
   const int32_t tc = static_cast<int32_t>(bid % A.tiles_c);
   bid /= A.tiles_c;
@@ -83,7 +89,7 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
   const int tid = threadIdx.x;
   const int lane = tid & 63;
 
-  for (int e = tid; e < kTS * kTB * kTCP; e += kThreads) acc[e] = 0.f;
+  for (int e = tid; e < kTS * kTBH * kTCP; e += kThreads) acc[e] = 0.f;
   __syncthreads();
 
   for (int32_t q = 0; q < A.n_angles; ++q) {
@@ -97,9 +103,12 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
     bool ok;
     {
       const int cl = lane & 7;
-      const float xa = static_cast<float>(cl & 1 ? i0 + ns - 1 : i0) + P.sig;
-      const float xb = static_cast<float>(cl & 2 ? j0 + nbt + 1 : j0) - P.taub;
-      const float xc = static_cast<float>(cl & 4 ? k0 + nct + 1 : k0) - P.tauc;
+      const double xa =
+          static_cast<double>(cl & 1 ? i0 + ns - 1 : i0) + P.sig;
+      const double xb =
+          static_cast<double>(cl & 2 ? j0 + nbt + 1 : j0) - P.taub;
+      const double xc =
+          static_cast<double>(cl & 4 ? k0 + nct + 1 : k0) - P.tauc;
       float c, r;
       project(P, xa, xb, xc, c, r, ok);
       cmin = cmax = c;
@@ -137,52 +146,39 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
     if (c_lo > c_hi || r_lo > r_hi) continue;
     const int32_t W = c_hi - c_lo + 1;
     const int32_t npx = W * (r_hi - r_lo + 1);
-    const float nb1 = static_cast<float>(A.nb + 1);
-    const float nc1 = static_cast<float>(A.nc + 1);
+    const double nb1 = static_cast<double>(A.nb + 1);
+    const double nc1 = static_cast<double>(A.nc + 1);
     for (int32_t t = tid; t < npx; t += kThreads) {
       const int32_t row = r_lo + t / W;
       const int32_t col = c_lo + t % W;
-      // ray parameters exactly as in the forward kernel
-      const float fr = static_cast<float>(row);
-      const float fcl = static_cast<float>(col);
-      const float Da = fmaf(fr, P.va, fmaf(fcl, P.ua, P.qa));
-      const float Db = fmaf(fr, P.vb, fmaf(fcl, P.ub, P.qb));
-      const float Dc = fmaf(fr, P.vc, fmaf(fcl, P.uc, P.qc));
-      const float a1 = Db / Da;
-      const float a2 = Dc / Da;
-      const float b1 = fmaf(a1, P.sig, P.taub);
-      const float b2 = fmaf(a2, P.sig, P.tauc);
-      if (!(isfinite(a1) && isfinite(a2) && isfinite(b1) && isfinite(b2)))
-        continue;
-      const float wscale = sqrtf(fmaf(a1 * a1, P.sc1,
-                                      fmaf(a2 * a2, P.sc2, 1.f))) * P.oscale;
-      const float val =
-          A.proj[ang * A.ps0 + row * A.ps1 + col * A.ps2] * wscale;
+      // ray parameters and samples exactly as in the forward kernel
+      double a1, b1, a2, b2;
+      if (!kk_fp_ray(P, row, col, a1, b1, a2, b2)) continue;
+      const float val = A.proj[ang * A.ps0 + row * A.ps1 + col * A.ps2] *
+                        kk_fp_weight(P, a1, a2);
       if (val == 0.f) continue;
       for (int32_t s = 0; s < ns; ++s) {
-        const float fi = static_cast<float>(i0 + s);
-        const float xb = kk_clamp(fmaf(a1, fi, b1), 0.f, nb1);
-        const float xc = kk_clamp(fmaf(a2, fi, b2), 0.f, nc1);
-        const float fb = __builtin_amdgcn_fractf(xb);
-        const float fc = __builtin_amdgcn_fractf(xc);
-        // local unpadded index of the first tap along b and c
-        const int32_t jb = static_cast<int32_t>(static_cast<uint32_t>(xb)) -
-                           kk::kHaloLo - j0;
-        const int32_t jc = static_cast<int32_t>(static_cast<uint32_t>(xc)) -
-                           kk::kHaloLo - k0;
-        if (jb >= nbt || jb < -1 || jc >= nct || jc < -1) continue;
+        const double fi = static_cast<double>(i0 + s);
+        uint32_t ixb, ixc;
+        float fb, fc;
+        kk_fp_coord(a1, b1, fi, nb1, ixb, fb);
+        kk_fp_coord(a2, b2, fi, nc1, ixc, fc);
+        // tile-local index of the taps along b and c, +1 for the halo; a
+        // tap outside the tile is clamped into the halo (index 0 or
+        // kTB + 1 / kTC + 1), which is never written back (valid voxels are
+        // 1..nbt, 1..nct)
+        const int32_t ub = static_cast<int32_t>(ixb) - kk::kHaloLo - j0 + 1;
+        const int32_t uc = static_cast<int32_t>(ixc) - kk::kHaloLo - k0 + 1;
+        const int32_t tb0 = min(max(ub, 0), kTB + 1);
+        const int32_t tb1 = min(max(ub + 1, 0), kTB + 1);
+        const int32_t tc0 = min(max(uc, 0), kTC + 1);
+        const int32_t tc1 = min(max(uc + 1, 0), kTC + 1);
         const float vb0 = val * (1.f - fb), vb1 = val * fb;
-        float* base = acc + s * (kTB * kTCP);
-        const bool b0 = jb >= 0, b1ok = jb + 1 < nbt;
-        const bool c0 = jc >= 0, c1ok = jc + 1 < nct;
-        if (c0) {
-          if (b0) atomicAdd(base + jb * kTCP + jc, vb0 * (1.f - fc));
-          if (b1ok) atomicAdd(base + (jb + 1) * kTCP + jc, vb1 * (1.f - fc));
-        }
-        if (c1ok) {
-          if (b0) atomicAdd(base + jb * kTCP + jc + 1, vb0 * fc);
-          if (b1ok) atomicAdd(base + (jb + 1) * kTCP + jc + 1, vb1 * fc);
-        }
+        float* base = acc + s * (kTBH * kTCP);
+        atomicAdd(base + tb0 * kTCP + tc0, vb0 * (1.f - fc));
+        atomicAdd(base + tb1 * kTCP + tc0, vb1 * (1.f - fc));
+        atomicAdd(base + tb0 * kTCP + tc1, vb0 * fc);
+        atomicAdd(base + tb1 * kTCP + tc1, vb1 * fc);
       }
     }
   }
@@ -194,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
     const int32_t r = e - s * plane;
     const int32_t j = r / nct;
     const int32_t k = r - j * nct;
-    const float v = acc[(s * kTB + j) * kTCP + k];
+    const float v = acc[(s * kTBH + j + 1) * kTCP + k + 1];
     const int64_t o = (i0 + s) * A.sa + (j0 + j) * A.sb + (k0 + k) * A.sc;
     if (A.mode == 1)
       A.vol[o] += v;

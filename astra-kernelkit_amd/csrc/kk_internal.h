@@ -35,27 +35,28 @@ constexpr int kNumXcd = 8;
 constexpr int kFrameB[3] = {1, 0, 1};
 constexpr int kFrameC[3] = {2, 2, 0};
 
-// Per-angle forward-projection record, float32, in the ray frame (a, b, c)
-// of that angle: a = ray (major) axis, (b, c) its frame's in-plane axes.
+// Per-angle forward-projection record in the ray frame (a, b, c) of that
+// angle: a = ray (major) axis, (b, c) its frame's in-plane axes.  Geometry
+// in float64: sample coordinates reach ~n + 1 (up to ~1025 voxels), where
+// float32 would round them by ~1e-4 voxel.
 struct alignas(16) FpAngle {
-  float qa, qb, qc, _p0;    // centre of pixel (0,0) minus source
-  float ua, ub, uc, _p1;    // column step U
-  float va, vb, vc, _p2;    // row step V
-  float sig, taub, tauc, _p3;  // slice i <-> in-plane coordinate offsets
-  float sc1, sc2, oscale, _p4; // sqrt(1 + a1^2 sc1 + a2^2 sc2) * oscale
+  double qa, qb, qc;        // centre of pixel (0,0) minus source
+  double ua, ub, uc;        // column step U
+  double va, vb, vc;        // row step V
+  double sig, taub, tauc;   // slice i <-> in-plane coordinate offsets
+  float sc1, sc2, oscale, _p0; // sqrt(1 + a1^2 sc1 + a2^2 sc2) * oscale
   int32_t na, nb, nc, axis;    // extents along a, b, c; world axis of a
   uint64_t tex_off;            // byte offset of the frame texture T_a
   uint32_t sab, sbb;           // slice / row byte strides of T_a
   int32_t colc;                // 1: V has no a/b part (a1, b1 row-free)
-  int32_t _p5[3];
+  int32_t _p1[3];
 };
 // Ray through pixel (r, c): D = q + c*U + r*V (relative to the source),
 // a1 = D_b / D_a, a2 = D_c / D_a.  Padded coordinate along b at slice i:
 // x_b(i) = a1 * (i + sig) + taub, with sig = 1/2 - n_a/2 - s_a and
-// taub = s_b + n_b/2 - 1/2 + KK_HALO_LO (both computed in float64 from the
-// volume-centred geometry, then rounded once), so x_b = j + f reads padded
+// taub = s_b + n_b/2 - 1/2 + KK_HALO_LO, so x_b = j + f reads padded
 // elements j and j + 1; 0 < x_b < n_b + 1 is the support of the volume.
-static_assert(sizeof(FpAngle) == 128, "FpAngle layout");
+static_assert(sizeof(FpAngle) == 160, "FpAngle layout");
 
 // Padded extents of one frame texture (elements of float2).
 inline int64_t frame_elems(const int32_t shape[3], int a) {
@@ -69,18 +70,18 @@ inline int64_t frame_elems(const int32_t shape[3], int a) {
 // taps at padded coordinate (jv, ju) are one 16-byte load.
 
 // Per-angle backprojection record: affine forms of the voxel centre X
-// (voxel frame, world order).  u = nu(X)/den(X) and v = nv(X)/den(X) are
-// padded texture coordinates on the detector (texel-centre 1/2 removed,
+// (voxel frame, world order), float64.  u = nu(X)/den(X) and v = nv(X)/den(X)
+// are padded texture coordinates on the detector (texel-centre 1/2 removed,
 // KK_HALO_LO added), 1/den^2 is the adjoint weight.  fast != 0 when nu and
 // den do not depend on z (circular orbit about z): then u and 1/den are
 // shared by a z-run.
 struct alignas(16) BpAngle {
-  float nux, nuy, nuz, nuw;
-  float nvx, nvy, nvz, nvw;
-  float dx, dy, dz, dw;
+  double nux, nuy, nuz, nuw;
+  double nvx, nvy, nvz, nvw;
+  double dx, dy, dz, dw;
   int32_t fast, _p0, _p1, _p2;
 };
-static_assert(sizeof(BpAngle) == 64, "BpAngle layout");
+static_assert(sizeof(BpAngle) == 112, "BpAngle layout");
 
 // One forward-projection problem of a launch: the plan's table read from
 // the frame texture `tex`, written to `out` (element strides os) as
@@ -219,3 +220,46 @@ __device__ __forceinline__ int32_t kk_wave_max(int32_t v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
   return __builtin_amdgcn_readfirstlane(v);
 }
+
+// ---- forward-projection ray arithmetic, shared by the forward kernel and
+// its exact transpose (both must compute identical samples) -------------
+
+// Slopes a1, a2 and intercepts b1, b2 of the padded sample coordinates
+// x_b(i) = a1 * i + b1, x_c(i) = a2 * i + b2 of the ray through pixel
+// (row, col) of angle P, in float64; false for rays parallel to the slices.
+__device__ __forceinline__ bool kk_fp_ray(const kk::FpAngle& P, int32_t row,
+                                          int32_t col, double& a1,
+                                          double& b1, double& a2,
+                                          double& b2) {
+  const double fr = static_cast<double>(row);
+  const double fc = static_cast<double>(col);
+  const double Da = fma(fr, P.va, fma(fc, P.ua, P.qa));
+  const double Db = fma(fr, P.vb, fma(fc, P.ub, P.qb));
+  const double Dc = fma(fr, P.vc, fma(fc, P.uc, P.qc));
+  const double inv = 1.0 / Da;
+  a1 = Db * inv;
+  a2 = Dc * inv;
+  b1 = fma(a1, P.sig, P.taub);
+  b2 = fma(a2, P.sig, P.tauc);
+  return isfinite(a1) && isfinite(a2) && isfinite(b1) && isfinite(b2);
+}
+
+// The ray's weight sqrt(1 + a1^2 sc1 + a2^2 sc2) * oscale.
+__device__ __forceinline__ float kk_fp_weight(const kk::FpAngle& P,
+                                              double a1, double a2) {
+  const float f1 = static_cast<float>(a1), f2 = static_cast<float>(a2);
+  return sqrtf(fmaf(f1 * f1, P.sc1, fmaf(f2 * f2, P.sc2, 1.f))) * P.oscale;
+}
+
+// Sample coordinate a * i + b clamped into [0, n1] (the zero halo), split
+// into the padded element index and the interpolation weight.  float64:
+// coordinates reach ~1e3, where float32 would be off by ~1e-4.
+__device__ __forceinline__ void kk_fp_coord(double a, double b, double fi,
+                                            double n1, uint32_t& idx,
+                                            float& frac) {
+  const double x = fmin(fmax(fma(a, fi, b), 0.0), n1);
+  const double fl = floor(x);
+  idx = static_cast<uint32_t>(fl);
+  frac = static_cast<float>(x - fl);
+}
+

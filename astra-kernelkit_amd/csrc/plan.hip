@@ -179,18 +179,18 @@ static int build_fp_table(const double* params, int32_t n,
     for (int t = 0; t < 3; ++t) q[t] = d0[t] + 0.5 * U[t] + 0.5 * V[t] - s[t];
     kk::FpAngle& e = tab[k];
     std::memset(&e, 0, sizeof(e));
-    e.qa = float(q[a]);
-    e.qb = float(q[b]);
-    e.qc = float(q[c]);
-    e.ua = float(U[a]);
-    e.ub = float(U[b]);
-    e.uc = float(U[c]);
-    e.va = float(V[a]);
-    e.vb = float(V[b]);
-    e.vc = float(V[c]);
-    e.sig = float(0.5 - 0.5 * shape[a] - s[a]);
-    e.taub = float(s[b] + 0.5 * shape[b] - 0.5 + kk::kHaloLo);
-    e.tauc = float(s[c] + 0.5 * shape[c] - 0.5 + kk::kHaloLo);
+    e.qa = q[a];
+    e.qb = q[b];
+    e.qc = q[c];
+    e.ua = U[a];
+    e.ub = U[b];
+    e.uc = U[c];
+    e.va = V[a];
+    e.vb = V[b];
+    e.vc = V[c];
+    e.sig = 0.5 - 0.5 * shape[a] - s[a];
+    e.taub = s[b] + 0.5 * shape[b] - 0.5 + kk::kHaloLo;
+    e.tauc = s[c] + 0.5 * shape[c] - 0.5 + kk::kHaloLo;
     e.sc1 = float(osc[a][0]);
     e.sc2 = float(osc[a][1]);
     e.oscale = float(osc[a][2]);
@@ -274,18 +274,18 @@ static int build_bp_table(const double* params, int32_t n,
                         k);
     kk::BpAngle& e = tab[k];
     std::memset(&e, 0, sizeof(e));
-    e.nux = float(nu[0]);
-    e.nuy = float(nu[1]);
-    e.nuz = float(nu[2]);
-    e.nuw = float(nu[3]);
-    e.nvx = float(nv[0]);
-    e.nvy = float(nv[1]);
-    e.nvz = float(nv[2]);
-    e.nvw = float(nv[3]);
-    e.dx = float(de[0]);
-    e.dy = float(de[1]);
-    e.dz = float(de[2]);
-    e.dw = float(de[3]);
+    e.nux = nu[0];
+    e.nuy = nu[1];
+    e.nuz = nu[2];
+    e.nuw = nu[3];
+    e.nvx = nv[0];
+    e.nvy = nv[1];
+    e.nvz = nv[2];
+    e.nvw = nv[3];
+    e.dx = de[0];
+    e.dy = de[1];
+    e.dz = de[2];
+    e.dw = de[3];
     // z-invariant column coordinate and weight: negligible (relative 1e-13)
     // z coefficients are treated as zero.
     const double du = std::fabs(nu[0]) + std::fabs(nu[1]) + std::fabs(nu[3]);
@@ -294,8 +294,8 @@ static int build_bp_table(const double* params, int32_t n,
                  ? 1
                  : 0;
     if (e.fast) {
-      e.nuz = 0.f;
-      e.dz = 0.f;
+      e.nuz = 0.0;
+      e.dz = 0.0;
     }
   }
   return KK_OK;

@@ -282,20 +282,20 @@ class Filter:
 
 # numpy views of the device records (csrc/kk_internal.h)
 FP_ANGLE_DTYPE = np.dtype([
-    ("qa", "<f4"), ("qb", "<f4"), ("qc", "<f4"), ("_p0", "<f4"),
-    ("ua", "<f4"), ("ub", "<f4"), ("uc", "<f4"), ("_p1", "<f4"),
-    ("va", "<f4"), ("vb", "<f4"), ("vc", "<f4"), ("_p2", "<f4"),
-    ("sig", "<f4"), ("taub", "<f4"), ("tauc", "<f4"), ("_p3", "<f4"),
-    ("sc1", "<f4"), ("sc2", "<f4"), ("oscale", "<f4"), ("_p4", "<f4"),
+    ("qa", "<f8"), ("qb", "<f8"), ("qc", "<f8"),
+    ("ua", "<f8"), ("ub", "<f8"), ("uc", "<f8"),
+    ("va", "<f8"), ("vb", "<f8"), ("vc", "<f8"),
+    ("sig", "<f8"), ("taub", "<f8"), ("tauc", "<f8"),
+    ("sc1", "<f4"), ("sc2", "<f4"), ("oscale", "<f4"), ("_p0", "<f4"),
     ("na", "<i4"), ("nb", "<i4"), ("nc", "<i4"), ("axis", "<i4"),
     ("tex_off", "<u8"), ("sab", "<u4"), ("sbb", "<u4"),
-    ("colc", "<i4"), ("_p5", "<i4", (3,))])
+    ("colc", "<i4"), ("_p1", "<i4", (3,))])
 BP_ANGLE_DTYPE = np.dtype([
-    ("nux", "<f4"), ("nuy", "<f4"), ("nuz", "<f4"), ("nuw", "<f4"),
-    ("nvx", "<f4"), ("nvy", "<f4"), ("nvz", "<f4"), ("nvw", "<f4"),
-    ("dx", "<f4"), ("dy", "<f4"), ("dz", "<f4"), ("dw", "<f4"),
+    ("nux", "<f8"), ("nuy", "<f8"), ("nuz", "<f8"), ("nuw", "<f8"),
+    ("nvx", "<f8"), ("nvy", "<f8"), ("nvz", "<f8"), ("nvw", "<f8"),
+    ("dx", "<f8"), ("dy", "<f8"), ("dz", "<f8"), ("dw", "<f8"),
     ("fast", "<i4"), ("_p0", "<i4", (3,))])
-assert FP_ANGLE_DTYPE.itemsize == 128 and BP_ANGLE_DTYPE.itemsize == 64
+assert FP_ANGLE_DTYPE.itemsize == 160 and BP_ANGLE_DTYPE.itemsize == 112
 
 
 def fp_table(params12: np.ndarray, ray_axes, rows, cols, shape,

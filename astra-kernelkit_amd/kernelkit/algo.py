@@ -68,10 +68,8 @@ def bp(projections: Any, projection_geometry, volume_geometry: VolumeGeometry,
     if is_device_array(out):
         out_dev = out
     else:
-        out_dev = torch.zeros(tuple(int(s) for s in volume_geometry.shape),
+        out_dev = torch.empty(tuple(int(s) for s in volume_geometry.shape),
                               dtype=torch.float32, device=default_device())
-        if isinstance(out, np.ndarray):
-            out_dev.copy_(torch.from_numpy(np.asarray(out, np.float32)))
     proj = as_device_array(projections)
     if preproc_fn is not None:
         if proj is projections:
@@ -94,7 +92,9 @@ def bp(projections: Any, projection_geometry, volume_geometry: VolumeGeometry,
     else:
         ptor.projections = proj
     ptor.volume = out_dev
-    ptor(additive=True)
+    # non-additive, as the reference's ``ptor()`` (its volume is zero-filled
+    # first, ``projector.py:586-587``)
+    ptor(additive=False)
     if is_device_array(out):
         return _to_host(out_dev)
     return _to_host(out_dev, out if isinstance(out, np.ndarray) else None)

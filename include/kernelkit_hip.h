@@ -188,8 +188,8 @@ int kk_batch_bp(const kk_batch* batch, int32_t grid_blocks, void* stream);
 /* ---- host-only introspection (no device calls) ---------------------- */
 
 /* The per-angle device records that kk_set_params_fp / kk_set_params_bp
- * would upload, written to `out` (n_angles x 128 bytes for the FP table,
- * n_angles x 64 bytes for the BP table; layouts FpAngle / BpAngle of
+ * would upload, written to `out` (n_angles x 160 bytes for the FP table,
+ * n_angles x 112 bytes for the BP table; layouts FpAngle / BpAngle of
  * csrc/kk_internal.h); frame_mask (nullable) receives the FP frame mask.
  * Used by CPU tests to check the table arithmetic. */
 int kk_fp_table(const double* params, int32_t n_angles,
