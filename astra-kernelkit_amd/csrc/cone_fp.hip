@@ -93,14 +93,18 @@ __device__ __forceinline__ float bilerp(kk_float4u t, float fb, float fc) {
 
 // One tracing pass over slices [i0, i1] (wave-uniform) for R rays of a
 // thread.  COLC: a1/b1 are shared by the R rays.
-template <int R, bool COLC>
+// One tracing pass over slices [i0, i1] (wave-uniform) for R rays of a
+// thread.  COLC: the b slope/intercept are shared by the R rays (nb1 = 1
+// entry in a1/b1).
+template <int R, int NB>
 __device__ __forceinline__ void trace(const kk::FpAngle& P,
                                       const char* __restrict__ frame,
-                                      const double (&a1)[R],
-                                      const double (&b1)[R],
+                                      const double (&a1)[NB],
+                                      const double (&b1)[NB],
                                       const double (&a2)[R],
                                       const double (&b2)[R], int32_t i0,
                                       int32_t i1, float (&acc)[R]) {
+  constexpr bool COLC = NB == 1;
   const double nb1 = static_cast<double>(P.nb + 1);
   const double nc1 = static_cast<double>(P.nc + 1);
   const uint32_t sab = P.sab, sbb = P.sbb;
@@ -133,10 +137,13 @@ __device__ __forceinline__ void trace(const kk::FpAngle& P,
 }
 
 // One workgroup's tile (row band rt, angle, column tile ct) of logical
-// block id `bid`.
-template <int R>
+// block id `bid`.  ALL_COLC: every angle of the launch is column-coherent,
+// so a thread keeps one b slope/intercept for its R rays (fewer registers,
+// more wavefronts in flight); otherwise the angle's flag picks the path.
+template <int R, bool ALL_COLC>
 __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
                                         const FpArgs& A, int64_t bid) {
+  constexpr int NB = ALL_COLC ? 1 : R;
   const int32_t ct = static_cast<int32_t>(bid % A.col_tiles);
   const int64_t t = bid / A.col_tiles;
   const int32_t ang = static_cast<int32_t>(t % A.n_angles);
@@ -150,26 +157,32 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
   const float nc1 = static_cast<float>(P.nc + 1);
   const float last = static_cast<float>(P.na - 1);
 
-  double a1[R], b1[R], a2[R], b2[R];
+  double a1[NB], b1[NB], a2[R], b2[R];
   float wscale[R];
   int32_t ilo = 0x7fffffff, ihi = -1;
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int32_t row = row0 + kWave * k;
     // rays outside the detector, or parallel to the slices, contribute
-    // nothing: park them in the halo
+    // nothing: park them in the halo (for column-coherent angles the b
+    // part of every ray of the thread equals that of its first ray)
+    double ra1, rb1;
     const bool ok = row < A.rows && col < A.cols &&
-                    kk_fp_ray(P, row, col, a1[k], b1[k], a2[k], b2[k]);
+                    kk_fp_ray(P, row, col, ra1, rb1, a2[k], b2[k]);
+    if (k < NB) {
+      a1[k] = ok ? ra1 : 0.0;
+      b1[k] = ok ? rb1 : -1.0;
+    }
     wscale[k] = 0.f;
     if (!ok) {
-      a1[k] = a2[k] = 0.0;
-      b1[k] = b2[k] = -1.0;
+      a2[k] = 0.0;
+      b2[k] = -1.0;
       continue;
     }
-    wscale[k] = kk_fp_weight(P, a1[k], a2[k]);
+    wscale[k] = kk_fp_weight(P, ra1, a2[k]);
     float lo = 0.f, hi = last;
-    clip_range(static_cast<float>(a1[k]), static_cast<float>(b1[k]), nb1,
-               last, lo, hi);
+    clip_range(static_cast<float>(ra1), static_cast<float>(rb1), nb1, last,
+               lo, hi);
     clip_range(static_cast<float>(a2[k]), static_cast<float>(b2[k]), nc1,
                last, lo, hi);
     if (lo <= hi + 2.f) {
@@ -185,10 +198,16 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
   for (int k = 0; k < R; ++k) acc[k] = 0.f;
   const char* frame = A.tex + P.tex_off;
   if (ilo <= ihi) {
-    if (P.colc)
-      trace<R, true>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
-    else
-      trace<R, false>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
+    if constexpr (ALL_COLC) {
+      trace<R, 1>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
+    } else {
+      if (P.colc) {
+        const double a1c[1] = {a1[0]}, b1c[1] = {b1[0]};
+        trace<R, 1>(P, frame, a1c, b1c, a2, b2, ilo, ihi, acc);
+      } else {
+        trace<R, R>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
+      }
+    }
   }
 
 #pragma unroll
@@ -211,19 +230,23 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
   }
 }
 
-template <int R>
-__global__ __launch_bounds__(256) void cone_fp_kernel(
+template <int R, bool ALL_COLC>
+__global__ __launch_bounds__(256, 8) void cone_fp_kernel(
     const kk::FpAngle* __restrict__ tab, FpArgs A) {
-  fp_tile<R>(tab, A, kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
+  fp_tile<R, ALL_COLC>(tab, A,
+                       kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
 }
 
 }  // namespace
 
 namespace {
 
+// Rows per thread: 2 unless the detector has at most 64 rows (measured on
+// MI355X: 2 beats 4 by 9-30% at 256^3..512^3, the float64 coordinates of
+// four rays costing occupancy).
 int auto_rows_per_thread(int32_t rows, int32_t rpt) {
   if (rpt == 1 || rpt == 2 || rpt == 4) return rpt;
-  return rows > 128 ? 4 : (rows > 64 ? 2 : 1);
+  return rows > 64 ? 2 : 1;
 }
 
 // Validate one forward-projection problem and fill its kernel arguments
@@ -280,8 +303,8 @@ struct FpItem {
   FpArgs args;
 };
 
-template <int R>
-__global__ __launch_bounds__(256) void cone_fp_batch_kernel(
+template <int R, bool ALL_COLC>
+__global__ __launch_bounds__(256, 8) void cone_fp_batch_kernel(
     const FpItem* __restrict__ items, const int64_t* __restrict__ first,
     int32_t n, int64_t total) {
   for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
@@ -295,8 +318,49 @@ __global__ __launch_bounds__(256) void cone_fp_batch_kernel(
     }
     const FpItem& it = items[lo];
     const int64_t local = g - first[lo];
-    fp_tile<R>(it.tab, it.args,
+    fp_tile<R, ALL_COLC>(it.tab, it.args,
                kk_xcd_remap(local, it.args.total_blocks, it.args.xcd_group));
+  }
+}
+
+template <bool C>
+void launch_single(int32_t rpt, dim3 grid, hipStream_t stream,
+                   const kk::FpAngle* tab, const FpArgs& A) {
+  const dim3 block(256);
+  switch (rpt) {
+    case 1:
+      hipLaunchKernelGGL((cone_fp_kernel<1, C>), grid, block, 0, stream, tab,
+                         A);
+      break;
+    case 2:
+      hipLaunchKernelGGL((cone_fp_kernel<2, C>), grid, block, 0, stream, tab,
+                         A);
+      break;
+    default:
+      hipLaunchKernelGGL((cone_fp_kernel<4, C>), grid, block, 0, stream, tab,
+                         A);
+      break;
+  }
+}
+
+template <bool C>
+void launch_batch(int32_t rpt, dim3 grid, hipStream_t stream,
+                  const FpItem* items, const int64_t* first, int32_t n,
+                  int64_t total) {
+  const dim3 block(256);
+  switch (rpt) {
+    case 1:
+      hipLaunchKernelGGL((cone_fp_batch_kernel<1, C>), grid, block, 0, stream,
+                         items, first, n, total);
+      break;
+    case 2:
+      hipLaunchKernelGGL((cone_fp_batch_kernel<2, C>), grid, block, 0, stream,
+                         items, first, n, total);
+      break;
+    default:
+      hipLaunchKernelGGL((cone_fp_batch_kernel<4, C>), grid, block, 0, stream,
+                         items, first, n, total);
+      break;
   }
 }
 
@@ -310,6 +374,7 @@ struct FpBatch {
   int32_t n = 0;
   int64_t total = 0;
   int32_t rpt = 4;
+  bool all_colc = false;
 };
 
 int fp_batch_build(const std::vector<FpSpec>& specs, int32_t rows_per_thread,
@@ -333,6 +398,8 @@ int fp_batch_build(const std::vector<FpSpec>& specs, int32_t rows_per_thread,
   b->n = static_cast<int32_t>(specs.size());
   b->total = total;
   b->rpt = rpt;
+  b->all_colc = true;
+  for (const FpSpec& S : specs) b->all_colc = b->all_colc && S.plan->fp_all_colc;
   hipError_t e = hipMalloc(&b->items, sizeof(FpItem) * items.size());
   if (e == hipSuccess) e = hipMalloc(&b->first, sizeof(int64_t) * first.size());
   if (e == hipSuccess)
@@ -363,20 +430,12 @@ int fp_batch_launch(const FpBatch* b, int32_t grid_blocks,
   if (g > b->total) g = b->total;
   if (g <= 0) return KK_OK;
   const dim3 grid(static_cast<unsigned>(g)), block(256);
-  switch (b->rpt) {
-    case 1:
-      hipLaunchKernelGGL(cone_fp_batch_kernel<1>, grid, block, 0, stream,
-                         b->items, b->first, b->n, b->total);
-      break;
-    case 2:
-      hipLaunchKernelGGL(cone_fp_batch_kernel<2>, grid, block, 0, stream,
-                         b->items, b->first, b->n, b->total);
-      break;
-    default:
-      hipLaunchKernelGGL(cone_fp_batch_kernel<4>, grid, block, 0, stream,
-                         b->items, b->first, b->n, b->total);
-      break;
-  }
+  if (b->all_colc)
+    launch_batch<true>(b->rpt, grid, stream, b->items, b->first, b->n,
+                       b->total);
+  else
+    launch_batch<false>(b->rpt, grid, stream, b->items, b->first, b->n,
+                        b->total);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "cone_fp_batch_kernel");
   return KK_OK;
@@ -444,21 +503,11 @@ int launch_fp_spec(const kk::FpSpec& S, int32_t rows_per_thread,
   FpArgs A;
   const int rc = make_fp_args(S, rpt, A);
   if (rc != KK_OK) return rc;
-  const dim3 grid(unsigned(A.total_blocks)), block(256);
-  switch (rpt) {
-    case 1:
-      hipLaunchKernelGGL(cone_fp_kernel<1>, grid, block, 0, stream,
-                         S.plan->fp, A);
-      break;
-    case 2:
-      hipLaunchKernelGGL(cone_fp_kernel<2>, grid, block, 0, stream,
-                         S.plan->fp, A);
-      break;
-    default:
-      hipLaunchKernelGGL(cone_fp_kernel<4>, grid, block, 0, stream,
-                         S.plan->fp, A);
-      break;
-  }
+  const dim3 grid(static_cast<unsigned>(A.total_blocks));
+  if (S.plan->fp_all_colc)
+    launch_single<true>(rpt, grid, stream, S.plan->fp, A);
+  else
+    launch_single<false>(rpt, grid, stream, S.plan->fp, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_kernel");
   return KK_OK;

@@ -3,37 +3,40 @@
 //
 // The reference has no such operator: its autograd backward uses the
 // voxel-driven backprojector (kernelkit/torch_support.py:29-33), which is
-// not the transpose of the forward projector.  Here every (ray, slice)
-// sample of the forward kernel, with the same arithmetic (kk_fp_ray,
-// kk_fp_weight, kk_fp_coord: ray slopes, intercepts, clamping into the zero
-// halo, tap index and weights), adds value * scale * bilinear weight to
-// each of its four taps that lie inside the volume, so
-// <A x, y> = <x, A^T y> up to rounding.
+// not the transpose of the forward projector.  Here every voxel sums, over
+// the (ray, slice) samples of the forward kernel that touch it, the ray's
+// value times the ray weight times the sample's bilinear weight for that
+// voxel.  Samples are recomputed with the forward kernel's arithmetic
+// (kk_fp_ray, kk_fp_weight, kk_fp_coord), so <A x, y> = <x, A^T y> up to
+// rounding.
 //
-// MI355X design: a scatter with one global float atomic per tap would put
-// ~4 atomics per sample on the L2.  Instead each workgroup owns a tile of
-// the volume in the ray frame of one axis (kTS slices x kTB x kTC voxels
-// plus a halo, 37 KiB of LDS) and accumulates it in LDS over every angle
-// of that axis: per angle it projects the tile's corners onto the detector,
-// walks the rays of the covering pixel rectangle through the tile's slices
-// and adds their taps with LDS float atomics (ds_add_f32).  The tile is
-// written to HBM once.  Angles of different ray axes run as separate
-// launches over tiles of their own frame, the later ones adding.
+// MI355X design: a gather, so that no atomics are needed (an LDS float
+// atomic add costs ~190 cycles per wavefront instruction on this part,
+// against ~8 for a plain LDS read-modify-write).  A lane owns kJB voxels
+// along b at one (slice, c) position, the 64 lanes of a wavefront
+// consecutive c positions, accumulated in registers over all angles of the
+// launch's ray axis and written once.  Per angle:
+// * circular orbits ("column-coherent" angles: the row step has no a/b
+//   part): the b coordinate of a sample depends on the detector column
+//   only, so the columns reaching the wavefront's b-run are a
+//   wave-uniform interval (inverse of the column -> b map).  The lanes
+//   compute 64 columns' ray data at a time into LDS; per column, the c
+//   coordinate is affine in the row, so each lane sums the rows reaching
+//   its voxel and the sum enters the b-run with the column's two b taps;
+// * other geometries: each lane projects the support rectangle of its
+//   voxels (at its slice) onto the detector and walks the pixels of the
+//   bounding box.
+// Candidate intervals are widened by 1e-6 voxel in coordinate space, far
+// above the float64 rounding of the sample coordinates; samples outside
+// the support get zero weight.
 #include "kk_internal.h"
 
 namespace {
 
-constexpr int kTS = 8;    // slices (ray axis a) per tile
-constexpr int kTB = 32;   // voxels along b per tile
-constexpr int kTC = 32;   // voxels along c per tile
-// The LDS tile has a one-voxel halo around the b and c extents: taps that
-// fall outside the tile are clamped into it and never written back, so the
-// sample loop has no bounds tests.  Row stride kTCP (odd): the lanes of a
-// wavefront (rays of consecutive detector columns, i.e. consecutive b rows)
-// hit different LDS banks.
-constexpr int kTBH = kTB + 2;
-constexpr int kTCP = kTC + 3;
-constexpr int kThreads = 256;
+constexpr int kJB = 8;       // voxels along b per lane
+constexpr int kWaves = 4;    // wavefronts per workgroup (b-runs)
+constexpr int kThreads = 64 * kWaves;
+constexpr double kEps = 1e-6;
 
 struct FptArgs {
   const float* proj;
@@ -48,13 +51,12 @@ struct FptArgs {
   int32_t mode;           // 0: vol = A^T y, 1: vol += A^T y
 };
 
-// Detector coordinates (col, row) in pixel-centre units of the ray from
-// the source through the source-relative point X (frame components): the
-// solution of q + col U + row V = lam X; ok iff finite with lam > 0.
-__device__ __forceinline__ void project(const kk::FpAngle& P, double xa,
-                                        double xb, double xc, float& col,
-                                        float& row, bool& ok) {
-  // V x X, q x X, V x q
+// Detector position (col, row) of the ray from the source through the
+// source-relative point X (frame components): q + col U + row V = lam X;
+// ok iff finite with lam > 0.
+__device__ __forceinline__ bool project(const kk::FpAngle& P, double xa,
+                                        double xb, double xc, double& col,
+                                        double& row) {
   const double vx0 = P.vb * xc - P.vc * xb;
   const double vx1 = P.vc * xa - P.va * xc;
   const double vx2 = P.va * xb - P.vb * xa;
@@ -65,137 +67,219 @@ __device__ __forceinline__ void project(const kk::FpAngle& P, double xa,
   const double vq1 = P.vc * P.qa - P.va * P.qc;
   const double vq2 = P.va * P.qb - P.vb * P.qa;
   const double den = P.ua * vx0 + P.ub * vx1 + P.uc * vx2;
-  const double c = -(P.qa * vx0 + P.qb * vx1 + P.qc * vx2) / den;
-  const double r = -(P.ua * qx0 + P.ub * qx1 + P.uc * qx2) / den;
+  col = -(P.qa * vx0 + P.qb * vx1 + P.qc * vx2) / den;
+  row = -(P.ua * qx0 + P.ub * qx1 + P.uc * qx2) / den;
   const double lam = (P.ua * vq0 + P.ub * vq1 + P.uc * vq2) / den;
-  ok = isfinite(c) && isfinite(r) && lam > 0.0 && fabs(c) < 1e8 &&
-       fabs(r) < 1e8;
-  col = static_cast<float>(ok ? c : 0.0);
-  row = static_cast<float>(ok ? r : 0.0);
+  return isfinite(col) && isfinite(row) && lam > 0.0;
 }
+
+// Integer interval [ceil(lo), floor(hi)] clipped to [0, n - 1].
+__device__ __forceinline__ void int_range(double lo, double hi, int32_t n,
+                                          int32_t& a, int32_t& b) {
+  lo = fmax(lo, -1.0);
+  hi = fmin(hi, static_cast<double>(n));
+  a = max(0, static_cast<int32_t>(ceil(lo)));
+  b = min(n - 1, static_cast<int32_t>(floor(hi)));
+}
+__device__ __forceinline__ void int_range(float lo, float hi, int32_t n,
+                                          int32_t& a, int32_t& b) {
+  lo = fmaxf(lo, -1.f);
+  hi = fminf(hi, static_cast<float>(n));
+  a = max(0, static_cast<int32_t>(ceilf(lo)));
+  b = min(n - 1, static_cast<int32_t>(floorf(hi)));
+}
+
+// Weight of voxel `v` (unpadded index) in a sample whose taps are the
+// padded element `idx` (voxels idx - 1 and idx, weights 1 - f and f).
+__device__ __forceinline__ float tap_weight(uint32_t idx, float f,
+                                            int32_t v) {
+  const int32_t t = static_cast<int32_t>(idx) - kk::kHaloLo;
+  return t == v ? 1.f - f : (t + 1 == v ? f : 0.f);
+}
+
+// Orders one wavefront's LDS accesses across its lanes (no workgroup
+// barrier: wavefronts leave the kernel independently).
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-column data of a column-coherent angle, computed lane-parallel and
+// read wave-uniformly from LDS.
+struct ColData {
+  double inv, a1, D0c;
+  float fb, alpha, beta;
+  int32_t t0;
+};
 
 __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
     const kk::FpAngle* __restrict__ tab, FptArgs A) {
-  __shared__ float acc[kTS * kTBH * kTCP];
   int64_t bid = blockIdx.x;
   const int32_t tc = static_cast<int32_t>(bid % A.tiles_c);
   bid /= A.tiles_c;
   const int32_t tb = static_cast<int32_t>(bid % A.tiles_b);
-  const int32_t ta = static_cast<int32_t>(bid / A.tiles_b);
-  const int32_t i0 = ta * kTS, j0 = tb * kTB, k0 = tc * kTC;
-  const int32_t ns = min(kTS, A.na - i0);
-  const int32_t nbt = min(kTB, A.nb - j0);
-  const int32_t nct = min(kTC, A.nc - k0);
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
+  const int32_t i = static_cast<int32_t>(bid / A.tiles_b);
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t k = tc * 64 + lane;
+  const int32_t jw = (tb * kWaves + static_cast<int32_t>(threadIdx.x >> 6)) *
+                     kJB;
+  if (jw >= A.nb) return;  // no barriers below: idle wavefronts may leave
 
-  for (int e = tid; e < kTS * kTBH * kTCP; e += kThreads) acc[e] = 0.f;
-  __syncthreads();
+  const double fi = static_cast<double>(i);
+  const double nb1 = static_cast<double>(A.nb + 1);
+  const double nc1 = static_cast<double>(A.nc + 1);
+  const double kd = static_cast<double>(k);  // general path
+  __shared__ ColData cols_lds[kWaves][64];
+  float acc[kJB];
+#pragma unroll
+  for (int t = 0; t < kJB; ++t) acc[t] = 0.f;
 
   for (int32_t q = 0; q < A.n_angles; ++q) {
     const int32_t ang = A.angles[q];
     const kk::FpAngle& P = tab[ang];
-    // Pixel rectangle covering the rays whose samples can reach the tile:
-    // samples at slices [i0, i0+ns) with padded coordinates in
-    // [j0, j0+nbt+1] x [k0, k0+nct+1].  Lanes 0..7 of each wave take one
-    // corner of that box each.
-    float cmin, cmax, rmin, rmax;
-    bool ok;
-    {
-      const int cl = lane & 7;
-      const double xa =
-          static_cast<double>(cl & 1 ? i0 + ns - 1 : i0) + P.sig;
-      const double xb =
-          static_cast<double>(cl & 2 ? j0 + nbt + 1 : j0) - P.taub;
-      const double xc =
-          static_cast<double>(cl & 4 ? k0 + nct + 1 : k0) - P.tauc;
-      float c, r;
-      project(P, xa, xb, xc, c, r, ok);
-      cmin = cmax = c;
-      rmin = rmax = r;
-#pragma unroll
-      for (int o = 1; o < 8; o <<= 1) {
-        cmin = fminf(cmin, __shfl_xor(cmin, o));
-        cmax = fmaxf(cmax, __shfl_xor(cmax, o));
-        rmin = fminf(rmin, __shfl_xor(rmin, o));
-        rmax = fmaxf(rmax, __shfl_xor(rmax, o));
-        ok = ok && __shfl_xor(static_cast<int>(ok), o);
+    const float* img = A.proj + ang * A.ps0;
+    const double arel = fi + P.sig;  // slice centre, relative to the source
+    if (P.colc) {
+      // columns whose b coordinate at this slice lies in
+      // [jw, jw + kJB + 1): a1 = (x_b - taub) / arel, and for these
+      // angles a1 = (col ub + qb) / (col ua + qa)
+      const double t1 = (jw - kEps - P.taub) / arel;
+      const double t2 = (jw + kJB + 1 + kEps - P.taub) / arel;
+      const double d1 = fma(t1, P.ua, -P.ub), d2 = fma(t2, P.ua, -P.ub);
+      int32_t c_lo = 0, c_hi = A.cols - 1;
+      if (d1 * d2 > 0.0) {
+        const double c1 = fma(-t1, P.qa, P.qb) / d1;
+        const double c2 = fma(-t2, P.qa, P.qb) / d2;
+        if (isfinite(c1) && isfinite(c2))
+          int_range(fmin(c1, c2), fmax(c1, c2), A.cols, c_lo, c_hi);
       }
-    }
-    int32_t c_lo = 0, c_hi = A.cols - 1, r_lo = 0, r_hi = A.rows - 1;
-    if (__builtin_amdgcn_readfirstlane(static_cast<int>(ok))) {
-      const float fcmin = __int_as_float(
-          __builtin_amdgcn_readfirstlane(__float_as_int(cmin)));
-      const float fcmax = __int_as_float(
-          __builtin_amdgcn_readfirstlane(__float_as_int(cmax)));
-      const float frmin = __int_as_float(
-          __builtin_amdgcn_readfirstlane(__float_as_int(rmin)));
-      const float frmax = __int_as_float(
-          __builtin_amdgcn_readfirstlane(__float_as_int(rmax)));
-      // one pixel of margin against rounding in the corner projection;
-      // the clamps keep the float -> int conversions in range
-      c_lo = max(c_lo, static_cast<int32_t>(floorf(kk_clamp(fcmin, -4.f,
-                                                            1e8f))) - 1);
-      c_hi = min(c_hi, static_cast<int32_t>(ceilf(kk_clamp(fcmax, -4.f,
-                                                           1e8f))) + 1);
-      r_lo = max(r_lo, static_cast<int32_t>(floorf(kk_clamp(frmin, -4.f,
-                                                            1e8f))) - 1);
-      r_hi = min(r_hi, static_cast<int32_t>(ceilf(kk_clamp(frmax, -4.f,
-                                                           1e8f))) + 1);
-    }
-    if (c_lo > c_hi || r_lo > r_hi) continue;
-    const int32_t W = c_hi - c_lo + 1;
-    const int32_t npx = W * (r_hi - r_lo + 1);
-    const double nb1 = static_cast<double>(A.nb + 1);
-    const double nc1 = static_cast<double>(A.nc + 1);
-    for (int32_t t = tid; t < npx; t += kThreads) {
-      const int32_t row = r_lo + t / W;
-      const int32_t col = c_lo + t % W;
-      // ray parameters and samples exactly as in the forward kernel
-      double a1, b1, a2, b2;
-      if (!kk_fp_ray(P, row, col, a1, b1, a2, b2)) continue;
-      const float val = A.proj[ang * A.ps0 + row * A.ps1 + col * A.ps2] *
-                        kk_fp_weight(P, a1, a2);
-      if (val == 0.f) continue;
-      for (int32_t s = 0; s < ns; ++s) {
-        const double fi = static_cast<double>(i0 + s);
-        uint32_t ixb, ixc;
-        float fb, fc;
-        kk_fp_coord(a1, b1, fi, nb1, ixb, fb);
-        kk_fp_coord(a2, b2, fi, nc1, ixc, fc);
-        // tile-local index of the taps along b and c, +1 for the halo; a
-        // tap outside the tile is clamped into the halo (index 0 or
-        // kTB + 1 / kTC + 1), which is never written back (valid voxels are
-        // 1..nbt, 1..nct)
-        const int32_t ub = static_cast<int32_t>(ixb) - kk::kHaloLo - j0 + 1;
-        const int32_t uc = static_cast<int32_t>(ixc) - kk::kHaloLo - k0 + 1;
-        const int32_t tb0 = min(max(ub, 0), kTB + 1);
-        const int32_t tb1 = min(max(ub + 1, 0), kTB + 1);
-        const int32_t tc0 = min(max(uc, 0), kTC + 1);
-        const int32_t tc1 = min(max(uc + 1, 0), kTC + 1);
-        const float vb0 = val * (1.f - fb), vb1 = val * fb;
-        float* base = acc + s * (kTBH * kTCP);
-        atomicAdd(base + tb0 * kTCP + tc0, vb0 * (1.f - fc));
-        atomicAdd(base + tb1 * kTCP + tc0, vb1 * (1.f - fc));
-        atomicAdd(base + tb0 * kTCP + tc1, vb0 * fc);
-        atomicAdd(base + tb1 * kTCP + tc1, vb1 * fc);
+      ColData* cd = cols_lds[threadIdx.x >> 6];
+      for (int32_t cb = c_lo; cb <= c_hi; cb += 64) {
+        const int32_t ncol = min(64, c_hi - cb + 1);
+        // the wavefront's columns, one per lane (exactly kk_fp_ray's Da,
+        // Db: their row terms are exact zeros for these angles)
+        if (lane < ncol) {
+          const double fc = static_cast<double>(cb + lane);
+          const double Da = fma(fc, P.ua, P.qa);
+          const double Db = fma(fc, P.ub, P.qb);
+          const double inv = 1.0 / Da;
+          const double a1 = Db * inv;
+          const double b1 = fma(a1, P.sig, P.taub);
+          ColData c;
+          c.t0 = -99;
+          if (isfinite(a1) && isfinite(b1)) {
+            uint32_t ib;
+            kk_fp_coord(a1, b1, fi, nb1, ib, c.fb);
+            c.t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
+          }
+          c.inv = inv;
+          c.a1 = a1;
+          c.D0c = fma(fc, P.uc, P.qc);
+          // c coordinate of the column's rays: alpha * row + beta
+          c.alpha = static_cast<float>(P.vc * inv * arel);
+          c.beta = static_cast<float>(fma(c.D0c * inv, arel, P.tauc));
+          cd[lane] = c;
+        }
+        wave_sync();
+        for (int32_t m = 0; m < ncol; ++m) {
+          const ColData c = cd[m];  // wave-uniform
+          if (c.t0 < -1 || c.t0 >= kJB) continue;
+          // rows whose c coordinate lies in [k, k + 2), found in float32
+          // and widened by 1e-2 row (float32 error ~1e-4 row); the
+          // samples themselves are evaluated exactly below
+          int32_t r_lo = 0, r_hi = A.rows - 1;
+          if (fabsf(c.alpha) > 1e-9f) {
+            const float ia = 1.f / c.alpha;
+            const float r1 = (static_cast<float>(k) - c.beta) * ia;
+            const float r2 = (static_cast<float>(k) + 2.f - c.beta) * ia;
+            int_range(fminf(r1, r2) - 1e-2f, fmaxf(r1, r2) + 1e-2f, A.rows,
+                      r_lo, r_hi);
+          }
+          const int32_t col = cb + m;
+          const float* ycol = img + col * A.ps2;
+          float colsum = 0.f;
+          for (int32_t r = r_lo; r <= r_hi; ++r) {
+            const double Dc = fma(static_cast<double>(r), P.vc, c.D0c);
+            const double a2 = Dc * c.inv;
+            const double b2 = fma(a2, P.sig, P.tauc);
+            if (!(isfinite(a2) && isfinite(b2))) continue;
+            uint32_t ic;
+            float fcw;
+            kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
+            const float wk = tap_weight(ic, fcw, k);
+            if (wk == 0.f) continue;
+            colsum = fmaf(ycol[r * A.ps1] * kk_fp_weight(P, c.a1, a2), wk,
+                          colsum);
+          }
+#pragma unroll
+          for (int t = 0; t < kJB; ++t) {
+            const float w =
+                t == c.t0 ? 1.f - c.fb : (t == c.t0 + 1 ? c.fb : 0.f);
+            acc[t] = fmaf(w, colsum, acc[t]);
+          }
+        }
+        wave_sync();  // reads done before the next batch's writes
+      }
+    } else {
+      // pixels whose rays meet the support rectangle of this lane's voxels
+      // at this slice: bounding box of its projected corners
+      double cmin = 1e300, cmax = -1e300, rmin = 1e300, rmax = -1e300;
+      bool ok = true;
+#pragma unroll
+      for (int corner = 0; corner < 4; ++corner) {
+        const double xb = (corner & 1 ? jw + kJB + 1 + kEps : jw - kEps) -
+                          P.taub;
+        const double xc = (corner & 2 ? kd + 2.0 + kEps : kd - kEps) -
+                          P.tauc;
+        double pc, pr;
+        ok = ok && project(P, arel, xb, xc, pc, pr);
+        cmin = fmin(cmin, pc);
+        cmax = fmax(cmax, pc);
+        rmin = fmin(rmin, pr);
+        rmax = fmax(rmax, pr);
+      }
+      int32_t c_lo = 0, c_hi = A.cols - 1, r_lo = 0, r_hi = A.rows - 1;
+      if (ok) {
+        int_range(cmin, cmax, A.cols, c_lo, c_hi);
+        int_range(rmin, rmax, A.rows, r_lo, r_hi);
+      }
+      for (int32_t r = r_lo; r <= r_hi; ++r) {
+        for (int32_t col = c_lo; col <= c_hi; ++col) {
+          double a1, b1, a2, b2;
+          if (!kk_fp_ray(P, r, col, a1, b1, a2, b2)) continue;
+          uint32_t ic;
+          float fcw;
+          kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
+          const float wk = tap_weight(ic, fcw, k);
+          if (wk == 0.f) continue;
+          uint32_t ib;
+          float fb;
+          kk_fp_coord(a1, b1, fi, nb1, ib, fb);
+          const int32_t t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
+          if (t0 < -1 || t0 >= kJB) continue;
+          const float val = img[r * A.ps1 + col * A.ps2] *
+                            kk_fp_weight(P, a1, a2) * wk;
+#pragma unroll
+          for (int t = 0; t < kJB; ++t) {
+            const float w = t == t0 ? 1.f - fb : (t == t0 + 1 ? fb : 0.f);
+            acc[t] = fmaf(w, val, acc[t]);
+          }
+        }
       }
     }
   }
-  __syncthreads();
 
-  const int32_t plane = nbt * nct;
-  for (int32_t e = tid; e < ns * plane; e += kThreads) {
-    const int32_t s = e / plane;
-    const int32_t r = e - s * plane;
-    const int32_t j = r / nct;
-    const int32_t k = r - j * nct;
-    const float v = acc[(s * kTBH + j + 1) * kTCP + k + 1];
-    const int64_t o = (i0 + s) * A.sa + (j0 + j) * A.sb + (k0 + k) * A.sc;
+  if (k >= A.nc) return;
+#pragma unroll
+  for (int t = 0; t < kJB; ++t) {
+    const int32_t j = jw + t;
+    if (j >= A.nb) break;
+    const int64_t o = i * A.sa + j * A.sb + k * A.sc;
     if (A.mode == 1)
-      A.vol[o] += v;
+      A.vol[o] += acc[t];
     else
-      A.vol[o] = v;
+      A.vol[o] = acc[t];
   }
 }
 
@@ -239,10 +323,9 @@ extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
     A.na = vol_shape[a];
     A.nb = vol_shape[b];
     A.nc = vol_shape[c];
-    A.tiles_b = (A.nb + kTB - 1) / kTB;
-    A.tiles_c = (A.nc + kTC - 1) / kTC;
-    const int64_t tiles_a = (A.na + kTS - 1) / kTS;
-    const int64_t blocks = tiles_a * A.tiles_b * A.tiles_c;
+    A.tiles_b = (A.nb + kJB * kWaves - 1) / (kJB * kWaves);
+    A.tiles_c = (A.nc + 63) / 64;
+    const int64_t blocks = int64_t(A.na) * A.tiles_b * A.tiles_c;
     if (blocks >= (int64_t(1) << 31))
       return kk::fail(KK_ERR_UNSUPPORTED, "kk_cone_fp_transpose: grid");
     A.angles = plan->fr_idx + plan->fr_start[a];
