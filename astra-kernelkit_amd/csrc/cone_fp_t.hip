@@ -10,22 +10,29 @@
 // (kk_fp_ray, kk_fp_weight, kk_fp_coord), so <A x, y> = <x, A^T y> up to
 // rounding.
 //
-// MI355X design: a gather, so that no atomics are needed (an LDS float
-// atomic add costs ~190 cycles per wavefront instruction on this part,
-// against ~8 for a plain LDS read-modify-write).  A lane owns kJB voxels
-// along b at one (slice, c) position, the 64 lanes of a wavefront
-// consecutive c positions, accumulated in registers over all angles of the
-// launch's ray axis and written once.  Per angle:
-// * circular orbits ("column-coherent" angles: the row step has no a/b
-//   part): the b coordinate of a sample depends on the detector column
-//   only, so the columns reaching the wavefront's b-run are a
-//   wave-uniform interval (inverse of the column -> b map).  The lanes
-//   compute 64 columns' ray data at a time into LDS; per column, the c
-//   coordinate is affine in the row, so each lane sums the rows reaching
-//   its voxel and the sum enters the b-run with the column's two b taps;
-// * other geometries: each lane projects the support rectangle of its
-//   voxels (at its slice) onto the detector and walks the pixels of the
-//   bounding box.
+// MI355X design, two launches per ray axis group:
+// 1. ray_weight_kernel: y * (ray weight), 0 for rays the forward kernel
+//    skips, written column-major ([angle][col][row]) into the caller's
+//    workspace through an LDS tile.  The gather below then reads, for one
+//    column, the rows of 64 neighbouring voxels: a few cache lines per
+//    wavefront instead of 64.
+// 2. cone_fp_t_kernel: a gather, so that no atomics are needed (an LDS
+//    float atomic add costs ~190 cycles per wavefront instruction on this
+//    part, against ~8 for a plain LDS read-modify-write).  A lane owns kJB
+//    voxels along b at one (slice, c) position, the 64 lanes of a
+//    wavefront consecutive c positions, accumulated in registers over all
+//    angles of the launch's ray axis and written once.  Per angle:
+//    * circular orbits ("column-coherent" angles: the row step has no a/b
+//      part): the b coordinate of a sample depends on the detector column
+//      only, so the columns reaching the wavefront's b-run are a
+//      wave-uniform interval (inverse of the column -> b map).  The lanes
+//      compute 64 columns' ray data at a time and broadcast them with
+//      readlane (scalar registers); per column, the c coordinate is affine
+//      in the row, so each lane sums the rows reaching its voxel and the
+//      sum enters the b-run with the column's two b taps (wave-uniform);
+//    * other geometries: each lane projects the support rectangle of its
+//      voxels (at its slice) onto the detector and walks the pixels of the
+//      bounding box.
 // Candidate intervals are widened by 1e-6 voxel in coordinate space, far
 // above the float64 rounding of the sample coordinates; samples outside
 // the support get zero weight.
@@ -37,10 +44,10 @@ constexpr int kJB = 8;       // voxels along b per lane
 constexpr int kWaves = 4;    // wavefronts per workgroup (b-runs)
 constexpr int kThreads = 64 * kWaves;
 constexpr double kEps = 1e-6;
+constexpr int kRU = 4;       // rows per load group (column-coherent path)
 
 struct FptArgs {
-  const float* proj;
-  int64_t ps0, ps1, ps2;  // projection strides: angle, row, col
+  const float* yw;        // weighted projections [angle][col][row]
   float* vol;
   int64_t sa, sb, sc;     // volume strides along the frame axes a, b, c
   int32_t rows, cols;
@@ -50,6 +57,45 @@ struct FptArgs {
   int32_t n_angles;
   int32_t mode;           // 0: vol = A^T y, 1: vol += A^T y
 };
+
+constexpr int kTile = 64;  // ray_weight_kernel: 64 x 64 pixels per block
+
+// yw[angle][col][row] = proj(angle, row, col) * kk_fp_weight of the ray,
+// 0 where kk_fp_ray rejects the ray.  Reads along columns and writes
+// along rows, both coalesced, through an LDS tile.
+__global__ __launch_bounds__(256) void ray_weight_kernel(
+    const kk::FpAngle* __restrict__ tab, const float* __restrict__ proj,
+    int64_t ps0, int64_t ps1, int64_t ps2, int32_t rows, int32_t cols,
+    int32_t tiles_r, int32_t tiles_c, float* __restrict__ yw) {
+  __shared__ float tile[kTile][kTile + 1];
+  int64_t bid = blockIdx.x;
+  const int32_t tcol = static_cast<int32_t>(bid % tiles_c);
+  bid /= tiles_c;
+  const int32_t trow = static_cast<int32_t>(bid % tiles_r);
+  const int32_t ang = static_cast<int32_t>(bid / tiles_r);
+  const kk::FpAngle& P = tab[ang];
+  const int32_t lane = threadIdx.x & 63;
+  const int32_t w = threadIdx.x >> 6;
+  const int32_t r0 = trow * kTile, c0 = tcol * kTile;
+  const float* img = proj + ang * ps0;
+  for (int32_t t = w; t < kTile; t += 4) {
+    const int32_t r = r0 + t, col = c0 + lane;
+    float v = 0.f;
+    if (r < rows && col < cols) {
+      double a1, b1, a2, b2;
+      if (kk_fp_ray(P, r, col, a1, b1, a2, b2))
+        v = img[r * ps1 + col * ps2] * kk_fp_weight(P, a1, a2);
+    }
+    tile[t][lane] = v;
+  }
+  __syncthreads();
+  float* out = yw + static_cast<int64_t>(ang) * rows * cols;
+  for (int32_t t = w; t < kTile; t += 4) {
+    const int32_t col = c0 + t, r = r0 + lane;
+    if (r < rows && col < cols)
+      out[static_cast<int64_t>(col) * rows + r] = tile[lane][t];
+  }
+}
 
 // Detector position (col, row) of the ray from the source through the
 // source-relative point X (frame components): q + col U + row V = lam X;
@@ -97,21 +143,35 @@ __device__ __forceinline__ float tap_weight(uint32_t idx, float f,
   return t == v ? 1.f - f : (t + 1 == v ? f : 0.f);
 }
 
-// Orders one wavefront's LDS accesses across its lanes (no workgroup
-// barrier: wavefronts leave the kernel independently).
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+// Value of lane `l` of a wavefront (l wave-uniform), in scalar registers.
+__device__ __forceinline__ int32_t bcast(int32_t v, int32_t l) {
+  return __builtin_amdgcn_readlane(v, l);
+}
+__device__ __forceinline__ float bcast(float v, int32_t l) {
+  return __builtin_bit_cast(
+      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), l));
+}
+__device__ __forceinline__ double bcast(double v, int32_t l) {
+  const uint64_t u = __builtin_bit_cast(uint64_t, v);
+  const uint32_t lo = static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int32_t>(u), l));
+  const uint32_t hi = static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int32_t>(u >> 32), l));
+  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// Per-column data of a column-coherent angle, computed lane-parallel and
-// read wave-uniformly from LDS.
-struct ColData {
-  double inv, a1, D0c;
-  float fb, alpha, beta;
-  int32_t t0;
-};
+// acc[t0] += (1 - fb) s, acc[t0 + 1] += fb s for a wave-uniform t0 in
+// [-1, kJB): scalar branches, two FMAs.
+__device__ __forceinline__ void add_b_taps(float (&acc)[kJB], int32_t t0,
+                                           float fb, float s) {
+#pragma unroll
+  for (int t = -1; t < kJB; ++t) {
+    if (t == t0) {
+      if (t >= 0) acc[t] = fmaf(1.f - fb, s, acc[t]);
+      if (t + 1 < kJB) acc[t + 1] = fmaf(fb, s, acc[t + 1]);
+    }
+  }
+}
 
 __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
     const kk::FpAngle* __restrict__ tab, FptArgs A) {
@@ -129,8 +189,9 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
   const double fi = static_cast<double>(i);
   const double nb1 = static_cast<double>(A.nb + 1);
   const double nc1 = static_cast<double>(A.nc + 1);
-  const double kd = static_cast<double>(k);  // general path
-  __shared__ ColData cols_lds[kWaves][64];
+  const double kd = static_cast<double>(k);
+  const float kf = static_cast<float>(k);
+  const int64_t angle_elems = static_cast<int64_t>(A.rows) * A.cols;
   float acc[kJB];
 #pragma unroll
   for (int t = 0; t < kJB; ++t) acc[t] = 0.f;
@@ -138,7 +199,7 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
   for (int32_t q = 0; q < A.n_angles; ++q) {
     const int32_t ang = A.angles[q];
     const kk::FpAngle& P = tab[ang];
-    const float* img = A.proj + ang * A.ps0;
+    const float* img = A.yw + ang * angle_elems;
     const double arel = fi + P.sig;  // slice centre, relative to the source
     if (P.colc) {
       // columns whose b coordinate at this slice lies in
@@ -154,72 +215,70 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
         if (isfinite(c1) && isfinite(c2))
           int_range(fmin(c1, c2), fmax(c1, c2), A.cols, c_lo, c_hi);
       }
-      ColData* cd = cols_lds[threadIdx.x >> 6];
       for (int32_t cb = c_lo; cb <= c_hi; cb += 64) {
         const int32_t ncol = min(64, c_hi - cb + 1);
-        // the wavefront's columns, one per lane (exactly kk_fp_ray's Da,
-        // Db: their row terms are exact zeros for these angles)
+        // ray data of column cb + lane (exactly kk_fp_ray's: the row terms
+        // of a and b are exact zeros for these angles)
+        double inv = 0.0, D0c = 0.0;
+        float fb = 0.f, ia = 0.f, beta = 0.f;
+        int32_t t0 = -99;
         if (lane < ncol) {
           const double fc = static_cast<double>(cb + lane);
           const double Da = fma(fc, P.ua, P.qa);
           const double Db = fma(fc, P.ub, P.qb);
-          const double inv = 1.0 / Da;
+          inv = 1.0 / Da;
           const double a1 = Db * inv;
           const double b1 = fma(a1, P.sig, P.taub);
-          ColData c;
-          c.t0 = -99;
           if (isfinite(a1) && isfinite(b1)) {
             uint32_t ib;
-            kk_fp_coord(a1, b1, fi, nb1, ib, c.fb);
-            c.t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
+            kk_fp_coord(a1, b1, fi, nb1, ib, fb);
+            t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
           }
-          c.inv = inv;
-          c.a1 = a1;
-          c.D0c = fma(fc, P.uc, P.qc);
+          D0c = fma(fc, P.uc, P.qc);
           // c coordinate of the column's rays: alpha * row + beta
-          c.alpha = static_cast<float>(P.vc * inv * arel);
-          c.beta = static_cast<float>(fma(c.D0c * inv, arel, P.tauc));
-          cd[lane] = c;
+          const float alpha = static_cast<float>(P.vc * inv * arel);
+          ia = fabsf(alpha) > 1e-9f ? 1.f / alpha : 0.f;
+          beta = static_cast<float>(fma(D0c * inv, arel, P.tauc));
         }
-        wave_sync();
         for (int32_t m = 0; m < ncol; ++m) {
-          const ColData c = cd[m];  // wave-uniform
-          if (c.t0 < -1 || c.t0 >= kJB) continue;
+          const int32_t tm = bcast(t0, m);
+          if (tm < -1 || tm >= kJB) continue;
+          const double invm = bcast(inv, m), D0cm = bcast(D0c, m);
+          const float iam = bcast(ia, m);
           // rows whose c coordinate lies in [k, k + 2), found in float32
           // and widened by 1e-2 row (float32 error ~1e-4 row); the
           // samples themselves are evaluated exactly below
           int32_t r_lo = 0, r_hi = A.rows - 1;
-          if (fabsf(c.alpha) > 1e-9f) {
-            const float ia = 1.f / c.alpha;
-            const float r1 = (static_cast<float>(k) - c.beta) * ia;
-            const float r2 = (static_cast<float>(k) + 2.f - c.beta) * ia;
+          if (iam != 0.f) {
+            const float r1 = (kf - bcast(beta, m)) * iam;
+            const float r2 = fmaf(2.f, iam, r1);
             int_range(fminf(r1, r2) - 1e-2f, fmaxf(r1, r2) + 1e-2f, A.rows,
                       r_lo, r_hi);
           }
-          const int32_t col = cb + m;
-          const float* ycol = img + col * A.ps2;
+          const float* ycol = img + (cb + m) * A.rows;
           float colsum = 0.f;
-          for (int32_t r = r_lo; r <= r_hi; ++r) {
-            const double Dc = fma(static_cast<double>(r), P.vc, c.D0c);
-            const double a2 = Dc * c.inv;
-            const double b2 = fma(a2, P.sig, P.tauc);
-            if (!(isfinite(a2) && isfinite(b2))) continue;
-            uint32_t ic;
-            float fcw;
-            kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
-            const float wk = tap_weight(ic, fcw, k);
-            if (wk == 0.f) continue;
-            colsum = fmaf(ycol[r * A.ps1] * kk_fp_weight(P, c.a1, a2), wk,
-                          colsum);
-          }
+          // rows in groups of kRU, loads first (a lane's rows are
+          // contiguous: one or two groups), summed in row order
+          for (int32_t r = r_lo; r <= r_hi; r += kRU) {
+            float y[kRU];
 #pragma unroll
-          for (int t = 0; t < kJB; ++t) {
-            const float w =
-                t == c.t0 ? 1.f - c.fb : (t == c.t0 + 1 ? c.fb : 0.f);
-            acc[t] = fmaf(w, colsum, acc[t]);
+            for (int u = 0; u < kRU; ++u) y[u] = ycol[min(r + u, r_hi)];
+#pragma unroll
+            for (int u = 0; u < kRU; ++u) {
+              const double Dc = fma(static_cast<double>(r + u), P.vc, D0cm);
+              const double a2 = Dc * invm;
+              const double b2 = fma(a2, P.sig, P.tauc);
+              uint32_t ic;
+              float fcw;
+              kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
+              // rays the forward kernel skips have yw = 0 (and clamped,
+              // finite coordinates here)
+              if (r + u <= r_hi)
+                colsum = fmaf(y[u], tap_weight(ic, fcw, k), colsum);
+            }
           }
+          add_b_taps(acc, tm, bcast(fb, m), colsum);
         }
-        wave_sync();  // reads done before the next batch's writes
       }
     } else {
       // pixels whose rays meet the support rectangle of this lane's voxels
@@ -244,8 +303,9 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
         int_range(cmin, cmax, A.cols, c_lo, c_hi);
         int_range(rmin, rmax, A.rows, r_lo, r_hi);
       }
-      for (int32_t r = r_lo; r <= r_hi; ++r) {
-        for (int32_t col = c_lo; col <= c_hi; ++col) {
+      for (int32_t col = c_lo; col <= c_hi; ++col) {
+        const float* ycol = img + col * A.rows;
+        for (int32_t r = r_lo; r <= r_hi; ++r) {
           double a1, b1, a2, b2;
           if (!kk_fp_ray(P, r, col, a1, b1, a2, b2)) continue;
           uint32_t ic;
@@ -258,8 +318,7 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
           kk_fp_coord(a1, b1, fi, nb1, ib, fb);
           const int32_t t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
           if (t0 < -1 || t0 >= kJB) continue;
-          const float val = img[r * A.ps1 + col * A.ps2] *
-                            kk_fp_weight(P, a1, a2) * wk;
+          const float val = ycol[r] * wk;
 #pragma unroll
           for (int t = 0; t < kJB; ++t) {
             const float w = t == t0 ? 1.f - fb : (t == t0 + 1 ? fb : 0.f);
@@ -290,30 +349,43 @@ extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
                                     int32_t cols, float* vol,
                                     const int32_t vol_shape[3],
                                     const int64_t vs[3], int32_t additive,
-                                    void* stream) {
+                                    float* work, void* stream) {
   if (plan == nullptr || !plan->fp_set)
     return kk::fail(KK_ERR_NOT_SET,
                     "Please set the projection geometry with `set_params()`.");
   if (proj == nullptr || ps == nullptr || vol == nullptr ||
-      vol_shape == nullptr || vs == nullptr)
+      vol_shape == nullptr || vs == nullptr || work == nullptr)
     return kk::fail(KK_ERR_INVALID, "kk_cone_fp_transpose: null argument");
   if (rows <= 0 || cols <= 0)
     return kk::fail(KK_ERR_INVALID, "kk_cone_fp_transpose: bad detector");
+  if (int64_t(rows) * cols >= (int64_t(1) << 31))
+    return kk::fail(KK_ERR_UNSUPPORTED,
+                    "kk_cone_fp_transpose: detector of 2^31 pixels or more");
   for (int t = 0; t < 3; ++t)
     if (vol_shape[t] != plan->fp_shape[t])
       return kk::fail(KK_ERR_SHAPE,
                       "kk_cone_fp_transpose: volume shape differs from the "
                       "one the parameters were set for");
+  const hipStream_t st = static_cast<hipStream_t>(stream);
+  {
+    const int32_t tr = (rows + kTile - 1) / kTile;
+    const int32_t tcl = (cols + kTile - 1) / kTile;
+    const int64_t blocks = int64_t(plan->fp_n) * tr * tcl;
+    if (blocks >= (int64_t(1) << 31))
+      return kk::fail(KK_ERR_UNSUPPORTED, "kk_cone_fp_transpose: grid");
+    hipLaunchKernelGGL(ray_weight_kernel, dim3(unsigned(blocks)), dim3(256),
+                       0, st, plan->fp, proj, ps[0], ps[1], ps[2], rows, cols,
+                       tr, tcl, work);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return kk::hip_fail(e, "ray_weight_kernel");
+  }
   bool first = true;
   for (int a = 0; a < 3; ++a) {
     const int32_t n = plan->fr_start[a + 1] - plan->fr_start[a];
     if (n == 0) continue;
     const int b = kk::kFrameB[a], c = kk::kFrameC[a];
     FptArgs A{};
-    A.proj = proj;
-    A.ps0 = ps[0];
-    A.ps1 = ps[1];
-    A.ps2 = ps[2];
+    A.yw = work;
     A.vol = vol;
     A.sa = vs[a];
     A.sb = vs[b];
@@ -333,8 +405,7 @@ extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
     A.mode = (first && !additive) ? 0 : 1;
     first = false;
     hipLaunchKernelGGL(cone_fp_t_kernel, dim3(unsigned(blocks)),
-                       dim3(kThreads), 0, static_cast<hipStream_t>(stream),
-                       plan->fp, A);
+                       dim3(kThreads), 0, st, plan->fp, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_t_kernel");
   }

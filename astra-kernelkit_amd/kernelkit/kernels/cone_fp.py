@@ -373,8 +373,13 @@ class RayDrivenConeFP(BaseKernel):
         self._upload(rows, cols, volume_geometry)
         pst = _logical_strides(projections, self._proj_axs)
         vst = _logical_strides(volume, self._vol_axs)
+        # the library's scratch: weighted projections, column-major
+        work = torch.empty(n * rows * cols, dtype=torch.float32,
+                           device=projections.device)
+        if stream is not None:
+            work.record_stream(stream)
         _native.check(_native.lib().kk_cone_fp_transpose(
             self._plan.ptr, _native._vp(projections.data_ptr()),
             _native.i64(pst), rows, cols, _native._vp(volume.data_ptr()),
             _native.i32(world), _native.i64(vst), 1 if additive else 0,
-            _native.stream_ptr(stream)))
+            _native._vp(work.data_ptr()), _native.stream_ptr(stream)))

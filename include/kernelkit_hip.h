@@ -132,12 +132,14 @@ int kk_cone_bp(const kk_plan* plan, const float* proj_tex, int32_t n_angles,
  * backprojector (kernelkit/torch_support.py:29-33); it replaces that call
  * where an exact gradient is wanted.  proj: (angle, row, col) strides; vol:
  * world strides, shape equal to the parameters' volume shape.  additive !=
- * 0: vol += A^T proj, else vol = A^T proj. */
+ * 0: vol += A^T proj, else vol = A^T proj.  work: caller-owned device
+ * scratch of n_angles * rows * cols floats (overwritten; the weighted
+ * projections in column-major order).  rows * cols < 2^31. */
 int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
                          const int64_t proj_strides[3], int32_t rows,
                          int32_t cols, float* vol, const int32_t vol_shape[3],
                          const int64_t vol_strides[3], int32_t additive,
-                         void* stream);
+                         float* work, void* stream);
 
 /* ---- batches of independent problems (one launch per operator) ------ */
 

@@ -322,13 +322,14 @@ def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
         D0c = fc * g["uc"] + g["qc"]
         alpha = (g["vc"] * inv * arel).astype(f32)
         beta = (D0c * inv * arel + tauc).astype(f32)
+        steep = np.abs(alpha) > f32(1e-9)
         with np.errstate(divide="ignore", over="ignore"):
-            ia = f32(1) / alpha
+            ia = np.where(steep, f32(1) / alpha, f32(0))
             r1 = (K.astype(f32) - beta) * ia
-            r2 = (K.astype(f32) + f32(2) - beta) * ia
+            # fmaf(2, ia, r1): exact in float64, rounded once
+            r2 = (2.0 * ia.astype(np.float64) + r1).astype(f32)
         r_lo, r_hi = _int_range(np.minimum(r1, r2) - f32(1e-2),
                                 np.maximum(r1, r2) + f32(1e-2), rows)
-        steep = np.abs(alpha) > f32(1e-9)
         r_lo, r_hi = np.where(steep, r_lo, 0), np.where(steep, r_hi, rows - 1)
     else:
         cs, rs, oks = [], [], []
