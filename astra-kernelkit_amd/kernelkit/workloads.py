@@ -93,6 +93,39 @@ def ellipsoid_phantom(shape, n: int = 10, seed: int = 0,
     return vol
 
 
+def ellipsoid_phantom_device(shape, n: int = 10, seed: int = 0,
+                             device=None):
+    """:func:`ellipsoid_phantom` evaluated on a torch device (float64
+    quadratic forms over each ellipsoid's bounding box), for volumes too
+    large to build on the host.  Same ellipsoids; voxels within rounding of
+    an ellipsoid surface may differ from the host version."""
+    import torch
+    shape = tuple(int(s) for s in shape)
+    vol = torch.zeros(shape, dtype=torch.float32, device=device)
+    for e in random_ellipsoids(shape, n, seed):
+        r = float(e.semi_axes.max())
+        lo = np.maximum(np.floor(e.centre - r).astype(int), 0)
+        hi = np.minimum(np.ceil(e.centre + r).astype(int) + 1, shape)
+        if np.any(hi <= lo):
+            continue
+        inv = torch.tensor(e.rotation / e.semi_axes[None, :],
+                           dtype=torch.float64, device=device)
+        ax = [torch.arange(int(lo[d]), int(hi[d]), dtype=torch.float64,
+                           device=device) - float(e.centre[d])
+              for d in range(3)]
+        q = torch.zeros((len(ax[0]), len(ax[1]), len(ax[2])),
+                        dtype=torch.float64, device=device)
+        for k in range(3):
+            t = (ax[0][:, None, None] * inv[0, k]
+                 + ax[1][None, :, None] * inv[1, k]
+                 + ax[2][None, None, :] * inv[2, k])
+            q += t * t
+        vol[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] += torch.where(
+            q <= 1.0, float(e.density), 0.0).to(torch.float32)
+        del q
+    return vol
+
+
 def circular_geometry(n_angles: int, sod: float, odd: float, rows: int,
                       cols: int, pixel: float, wobble_deg: float = 0.0,
                       drift_px: float = 0.0, tilt_deg: float = 0.0,

@@ -281,6 +281,73 @@ def autograd_diff():
     }
 
 
+def large_unrolled(iters=5):
+    """Differentiation through ``iters`` unrolled SIRT iterations at the
+    largest workload (1024^3 volume, 1440 angles with seeded +-1 degree
+    wobble and +-3 pixel drift, 1536^2 detector): ``x_k`` from
+    ``torch_support.unrolled_reconstruction`` (exact transpose, SIRT
+    weights, one learnable step size per iteration, checkpointed
+    iterations), loss ``||x_iters - x_true||^2``, gradient to the step
+    sizes.  Reports the forward and backward times, the operator calls
+    they make, and the peak device memory held by tensors."""
+    import numpy as np
+    import torch
+
+    import kernelkit as kk
+    from kernelkit import workloads
+    from kernelkit.torch_support import unrolled_reconstruction
+
+    dev = torch.device("cuda", 0)
+    pgs, vg = workloads.config_geometry("large")
+    x_true = workloads.ellipsoid_phantom_device(vg.shape, n=10, seed=0,
+                                                device=dev)
+    A = kk.XrayTransform(pgs, vg)
+    stream = torch.cuda.current_stream(dev)
+    y = A(x_true)
+    t_fp = _event_ms(lambda: A(x_true, out=y), stream)
+    _log(f"large: forward projection {t_fp:.1f} ms")
+    steps = torch.ones(iters, dtype=torch.float64, device=dev,
+                       requires_grad=True)
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    base = torch.cuda.memory_allocated(dev)
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    c = torch.cuda.Event(enable_timing=True)
+    a.record(stream)
+    x = unrolled_reconstruction(y, A, iters=iters, step=steps,
+                                weighting="sirt", checkpoint=True)
+    loss = ((x - x_true) ** 2).sum()
+    b.record(stream)
+    _log("large: forward done")
+    loss.backward()
+    c.record(stream)
+    c.synchronize()
+    t_f, t_b = a.elapsed_time(b), b.elapsed_time(c)
+    peak = torch.cuda.max_memory_allocated(dev)
+    updates = int(np.prod(vg.shape)) * len(pgs)
+    # forward: R = 1/A1, W = 1/A^T 1, then A and A^T per iteration;
+    # backward: each iteration recomputed (A, A^T) and differentiated
+    # (A^T of A's output gradient, A of A^T's)
+    calls_f = 2 + 2 * iters
+    calls_b = 4 * iters
+    return {
+        "workload": ("1024^3 volume, 1440 angles (seeded +-1 deg wobble, "
+                     "+-3 px drift), 1536^2 detector; "
+                     f"{iters} unrolled SIRT iterations, checkpointed"),
+        "forward_projection_ms": round(t_fp, 1),
+        "unrolled_forward_ms": round(t_f, 1),
+        "backward_ms": round(t_b, 1),
+        "operator_calls": {"forward": calls_f, "backward": calls_b},
+        "gups": round(updates * (calls_f + calls_b) / ((t_f + t_b) * 1e-3)
+                      / 1e9, 1),
+        "peak_tensor_bytes": int(peak),
+        "peak_tensor_gb": round(peak / 1e9, 2),
+        "resident_before_gb": round(base / 1e9, 2),
+        "step_grad": [float(g) for g in steps.grad.cpu()],
+    }
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -292,6 +359,11 @@ def main(argv=None):
     ap.add_argument("--quick", action="store_true",
                     help="only the FP/BP step (no batched SIRT or autograd "
                          "comparison)")
+    ap.add_argument("--extras", default="batched,autograd,algorithms",
+                    help="comma-separated extra measurements after the "
+                         "FP/BP step at the full config: batched, "
+                         "autograd, algorithms, large (not in the "
+                         "default list: several minutes)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     metavar="MIB",
                     help="after the timed steps, run one plain device copy "
@@ -438,13 +510,26 @@ def main(argv=None):
         },
         "cpu_baseline": None,
     }
-    if not args.quick and args.config == "full":
+    extras = set() if args.quick or args.config != "full" else {
+        e.strip() for e in args.extras.split(",") if e.strip()}
+    unknown = extras - {"batched", "autograd", "algorithms", "large"}
+    if unknown:
+        print(f"unknown extras: {sorted(unknown)}", file=sys.stderr)
+        return 2
+    if "batched" in extras:
         _log("batched SIRT vs per-problem ...")
         out["batched"] = batched_sirt()
+    if "autograd" in extras:
         _log("autograd (exact transpose) vs plain ...")
         out["autograd"] = autograd_diff()
+    if "algorithms" in extras:
         _log("FDK and SIRT at full size ...")
         out["algorithms"] = algorithms_full(pgs, vg, x)
+    if "large" in extras:
+        _log("unrolled differentiation at 1024^3 ...")
+        del fptor, bptor, y, v
+        torch.cuda.empty_cache()
+        out["large"] = large_unrolled()
     if not args.no_cpu_baseline:
         _log("cpu baseline (FP64 oracle, 1 core) ...")
         out["cpu_baseline"] = cpu_baseline(pgs, vg, vol)

@@ -110,3 +110,16 @@ def test_filter_names_and_padding():
         _filter(64, "nope")
     with pytest.raises(ValueError):
         _filter(63)
+
+
+def test_device_phantom_matches_host_phantom():
+    """The device phantom builder (used for the 1024^3 workload) evaluates
+    the same ellipsoids as the host builder."""
+    import torch
+    from kernelkit import workloads
+    for shape, seed in (((40, 48, 36), 3), ((33, 20, 25), 7)):
+        host = workloads.ellipsoid_phantom(shape, n=6, seed=seed)
+        dev = workloads.ellipsoid_phantom_device(shape, n=6, seed=seed,
+                                                 device="cpu").numpy()
+        assert host.sum() > 0
+        np.testing.assert_array_equal(dev, host)
