@@ -14,8 +14,9 @@
 //   orbit about z the column coordinate u and the weight 1/den do not depend
 //   on z, so the reciprocal, u and its interpolation weights are computed
 //   once per (x, y, angle) and shared by the K voxels; rows advance by a
-//   constant step.  Other geometries take the general per-voxel path (a
-//   wave-uniform branch per angle);
+//   constant step.  A detector tilted in its own plane keeps 1/den shared
+//   (u and v both affine along the run); other geometries take the
+//   general per-voxel path (a wave-uniform branch per angle);
 // * projections are read from a zero-halo texture of (value, next-row)
 //   float2 pairs (kk_internal.h): a sample is a clamp, ONE 16-byte load for
 //   the four taps and three lerps, with no bounds tests (the
@@ -118,7 +119,43 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
     const double den = fma(Y, P.dy, fma(X, P.dx, P.dw));
     const double nu = fma(Y, P.nuy, fma(X, P.nux, P.nuw));
     const double nv = fma(Y, P.nvy, fma(X, P.nvx, P.nvw));
-    if (P.fast) {
+    if (P.fast == 2) {
+      // 1/den and the weight do not depend on z; u and v are affine in z
+      const float rf = __builtin_amdgcn_rcpf(static_cast<float>(den));
+      const double r = refine_rcp(den, rf);
+      const double u0 = fma(Z0, P.nuz, nu) * r;
+      const double v0 = fma(Z0, P.nvz, nv) * r;
+      const float gu = static_cast<float>(P.nuz * r);
+      const float gv = static_cast<float>(P.nvz * r);
+      if (!(isfinite(u0) && isfinite(v0) && isfinite(gu) && isfinite(gv)))
+        continue;
+      const float w = rf * rf;
+      const double uc0 = fmin(fmax(u0, -1e6), 1e6);
+      const double ufl = floor(uc0);
+      const int32_t iub = static_cast<int32_t>(ufl);
+      const float fu0 = static_cast<float>(uc0 - ufl);
+      const double vc0 = fmin(fmax(v0, -1e6), 1e6);
+      const double vfl = floor(vc0);
+      const int32_t ivb = static_cast<int32_t>(vfl);
+      const float fv0 = static_cast<float>(vc0 - vfl);
+      uint32_t off[K];
+      float fu[K], fv[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const float fk = static_cast<float>(k);
+        int32_t iu, iv;
+        split_clamped(fmaf(fk, gu, fu0), iub, cmax, iu, fu[k]);
+        split_clamped(fmaf(fk, gv, fv0), ivb, rmax, iv, fv[k]);
+        off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
+                 (static_cast<uint32_t>(iu) << 3);
+      }
+      kk_float4u tp[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) tp[k] = kk_ld4(img, off[k]);
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        acc[k] = fmaf(w, bilerp(tp[k], fu[k], fv[k]), acc[k]);
+    } else if (P.fast) {
       // u and 1/den do not depend on z; v is affine in z
       const float rf = __builtin_amdgcn_rcpf(static_cast<float>(den));
       const double r = refine_rcp(den, rf);

@@ -32,7 +32,9 @@
 //      sum enters the b-run with the column's two b taps (wave-uniform);
 //    * other geometries: each lane projects the support rectangle of its
 //      voxels (at its slice) onto the detector and walks the pixels of the
-//      bounding box.
+//      bounding box; the b taps, whose index differs per lane, are summed
+//      in the lane's own LDS words (one read-modify-write pair per sample
+//      instead of a select over all kJB registers).
 // Candidate intervals are widened by 1e-6 voxel in coordinate space, far
 // above the float64 rounding of the sample coordinates; samples outside
 // the support get zero weight.
@@ -195,6 +197,13 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
   float acc[kJB];
 #pragma unroll
   for (int t = 0; t < kJB; ++t) acc[t] = 0.f;
+  // general path: b-tap sums of this lane at t = -1 .. kJB (row t + 1),
+  // indexed per lane; every lane owns its words, so no atomics and no
+  // cross-lane ordering are needed
+  __shared__ float gacc[kWaves][kJB + 2][64];
+  float(*g)[64] = gacc[threadIdx.x >> 6];
+#pragma unroll
+  for (int t = 0; t < kJB + 2; ++t) g[t][lane] = 0.f;
 
   for (int32_t q = 0; q < A.n_angles; ++q) {
     const int32_t ang = A.angles[q];
@@ -319,16 +328,15 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
           const int32_t t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
           if (t0 < -1 || t0 >= kJB) continue;
           const float val = ycol[r] * wk;
-#pragma unroll
-          for (int t = 0; t < kJB; ++t) {
-            const float w = t == t0 ? 1.f - fb : (t == t0 + 1 ? fb : 0.f);
-            acc[t] = fmaf(w, val, acc[t]);
-          }
+          g[t0 + 1][lane] = fmaf(1.f - fb, val, g[t0 + 1][lane]);
+          g[t0 + 2][lane] = fmaf(fb, val, g[t0 + 2][lane]);
         }
       }
     }
   }
 
+#pragma unroll
+  for (int t = 0; t < kJB; ++t) acc[t] += g[t + 1][lane];
   if (k >= A.nc) return;
 #pragma unroll
   for (int t = 0; t < kJB; ++t) {

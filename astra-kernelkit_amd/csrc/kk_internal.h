@@ -72,9 +72,11 @@ inline int64_t frame_elems(const int32_t shape[3], int a) {
 // Per-angle backprojection record: affine forms of the voxel centre X
 // (voxel frame, world order), float64.  u = nu(X)/den(X) and v = nv(X)/den(X)
 // are padded texture coordinates on the detector (texel-centre 1/2 removed,
-// KK_HALO_LO added), 1/den^2 is the adjoint weight.  fast != 0 when nu and
+// KK_HALO_LO added), 1/den^2 is the adjoint weight.  fast = 1 when nu and
 // den do not depend on z (circular orbit about z): then u and 1/den are
-// shared by a z-run.
+// shared by a z-run; fast = 2 when only den does not (detector tilted in
+// its plane): 1/den is shared and u, v are affine along the run; fast = 0
+// otherwise (per-voxel 1/den).
 struct alignas(16) BpAngle {
   double nux, nuy, nuz, nuw;
   double nvx, nvy, nvz, nvw;

@@ -289,17 +289,15 @@ static int build_bp_table(const double* params, int32_t n,
     e.dy = de[1];
     e.dz = de[2];
     e.dw = de[3];
-    // z-invariant column coordinate and weight: negligible (relative 1e-13)
-    // z coefficients are treated as zero.
+    // z-invariant weight (and column coordinate): negligible (relative
+    // 1e-13) z coefficients are treated as zero.
     const double du = std::fabs(nu[0]) + std::fabs(nu[1]) + std::fabs(nu[3]);
     const double dd = std::fabs(de[0]) + std::fabs(de[1]) + std::fabs(de[3]);
-    e.fast = (std::fabs(nu[2]) <= 1e-13 * du && std::fabs(de[2]) <= 1e-13 * dd)
-                 ? 1
-                 : 0;
-    if (e.fast) {
-      e.nuz = 0.0;
-      e.dz = 0.0;
-    }
+    const bool den_flat = std::fabs(de[2]) <= 1e-13 * dd;
+    const bool u_flat = std::fabs(nu[2]) <= 1e-13 * du;
+    e.fast = den_flat ? (u_flat ? 1 : 2) : 0;
+    if (den_flat) e.dz = 0.0;
+    if (e.fast == 1) e.nuz = 0.0;
   }
   return KK_OK;
 }

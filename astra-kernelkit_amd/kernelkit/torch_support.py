@@ -120,7 +120,7 @@ def _apply(fn, x, op):
 def unrolled_reconstruction(y, op: XrayTransform, iters: int = 5, x0=None,
                             step=None, weighting: str = "sirt",
                             min_constraint: float | None = None,
-                            checkpoint: bool = True):
+                            checkpoint: bool = True, callback=None):
     """Differentiable unrolled reconstruction ``x_{k+1} = P(x_k - s_k W
     A^T(R (A x_k - y)))`` with the exact transpose ``A^T``.
 
@@ -133,7 +133,9 @@ def unrolled_reconstruction(y, op: XrayTransform, iters: int = 5, x0=None,
     ``torch.utils.checkpoint``: its intermediate projections and volumes
     are recomputed in the backward pass instead of being stored, so the
     memory held for the backward pass is one volume per iteration.
-    Gradients flow to ``y``, ``x0`` and ``step``.
+    Gradients flow to ``y``, ``x0`` and ``step``.  ``callback(k, x_k)``,
+    when given, is called after iteration ``k`` (as ``algo.sirt``'s
+    callback, ``algo.py:363-364``).
     """
     from torch.utils.checkpoint import checkpoint as _ckpt
     dev = y.device
@@ -180,4 +182,6 @@ def unrolled_reconstruction(y, op: XrayTransform, iters: int = 5, x0=None,
             x = _ckpt(one, x, y, s, use_reentrant=False)
         else:
             x = one(x, y, s)
+        if callback is not None:
+            callback(k, x)
     return x

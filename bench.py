@@ -314,9 +314,25 @@ def large_unrolled(iters=5):
     a = torch.cuda.Event(enable_timing=True)
     b = torch.cuda.Event(enable_timing=True)
     c = torch.cuda.Event(enable_timing=True)
+    t_start = time.perf_counter()
+
+    def progress(k, xk):
+        torch.cuda.synchronize()
+        _log(f"large: iteration {k} forward done "
+             f"({time.perf_counter() - t_start:.1f} s)")
+
+        def back(g):
+            # the gradient of x_k is ready once the GPU has run the
+            # backward pass of the iterations after k
+            torch.cuda.synchronize()
+            _log(f"large: backward reached iteration {k} "
+                 f"({time.perf_counter() - t_start:.1f} s)")
+        xk.register_hook(back)
+
     a.record(stream)
     x = unrolled_reconstruction(y, A, iters=iters, step=steps,
-                                weighting="sirt", checkpoint=True)
+                                weighting="sirt", checkpoint=True,
+                                callback=progress)
     loss = ((x - x_true) ** 2).sum()
     b.record(stream)
     _log("large: forward done")

@@ -121,7 +121,27 @@ def emulate_bp(proj, table, shape, scale):
         nu = Y * g["nuy"] + (X * g["nux"] + g["nuw"])
         nv = Y * g["nvy"] + (X * g["nvx"] + g["nvw"])
         with np.errstate(divide="ignore", invalid="ignore"):
-            if P["fast"]:
+            if P["fast"] == 2:
+                r = 1.0 / den
+                u0 = (Z0 * g["nuz"] + nu) * r
+                v0 = (Z0 * g["nvz"] + nv) * r
+                gu = (g["nuz"] * r).astype(f32)
+                gv = (g["nvz"] * r).astype(f32)
+                ok = (np.isfinite(u0) & np.isfinite(v0) & np.isfinite(gu)
+                      & np.isfinite(gv))
+                rf = (f32(1) / den.astype(f32)).astype(f32)
+                w = np.where(ok, rf * rf, f32(0))
+                uc0 = np.clip(np.where(ok, u0, 0.0), -1e6, 1e6)
+                ufl = np.floor(uc0)
+                vc0 = np.clip(np.where(ok, v0, 0.0), -1e6, 1e6)
+                vfl = np.floor(vc0)
+                iu, fu = _split_clamped(
+                    kz * np.where(ok, gu, f32(0)) + (uc0 - ufl).astype(f32),
+                    ufl.astype(np.int64), cols + 1)
+                iv, fv = _split_clamped(
+                    kz * np.where(ok, gv, f32(0)) + (vc0 - vfl).astype(f32),
+                    vfl.astype(np.int64), rows + 1)
+            elif P["fast"]:
                 r = 1.0 / den
                 u = nu * r
                 v0 = (Z0 * g["nvz"] + nv) * r
@@ -210,13 +230,17 @@ def test_fp_emulation_z_major():
     assert_parity(got, oracle.cone_fp(vol, pgs, vg), label="z-major")
 
 
-@pytest.mark.parametrize("pert", [{}, {"tilt_deg": 2.0, "drift_px": 4.0}])
-def test_bp_emulation_matches_oracle(pert):
+@pytest.mark.parametrize("pert,path", [
+    ({}, 1), ({"tilt_deg": 2.0, "drift_px": 4.0}, 2),
+    ({"wobble_deg": 1.0, "drift_px": 3.0}, 0)])
+def test_bp_emulation_matches_oracle(pert, path):
+    """All three paths: circular (shared u and 1/den), detector tilted in
+    its plane (shared 1/den, affine u and v), wobbling orbit (per voxel)."""
     pgs, vg, vol = _case(**pert)
     proj = oracle.cone_fp(vol, pgs, vg)
     B = kk.kernels.VoxelDrivenConeBP.geoms2params(pgs, vg)
     table = _native.bp_table(B)
-    assert bool(table["fast"].all()) == (not pert)
+    assert (table["fast"] == path).all()
     got = emulate_bp(proj, table, vg.shape, vg.voxel_volume())
     assert_parity(got, oracle.cone_bp(proj, pgs, vg), label=f"bp-emu {pert}")
 
