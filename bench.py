@@ -306,8 +306,10 @@ def large_unrolled(iters=5):
     y = A(x_true)
     t_fp = _event_ms(lambda: A(x_true, out=y), stream)
     _log(f"large: forward projection {t_fp:.1f} ms")
-    steps = torch.ones(iters, dtype=torch.float64, device=dev,
-                       requires_grad=True)
+    # distinct steps: with x0 = 0 the iteration maps (I - s_k W A^T R A)
+    # commute, so equal steps would give equal gradients for every k
+    steps = torch.linspace(1.0, 0.8, iters, dtype=torch.float64,
+                           device=dev).requires_grad_(True)
     torch.cuda.synchronize()
     torch.cuda.reset_peak_memory_stats(dev)
     base = torch.cuda.memory_allocated(dev)
