@@ -34,8 +34,6 @@
 // * per-angle parameters are wave-uniform (scalar loads); workgroups are
 //   ordered row band -> angle -> column tile, and grouped so that each XCD
 //   keeps the same column range across consecutive angles (L2 reuse).
-#include <algorithm>
-#include <cstdlib>
 #include <new>
 #include <vector>
 
@@ -60,7 +58,6 @@ struct FpArgs {
   int32_t mode;  // 0: out = p, 1: out += p, 2: out = (p - y) * R
   int32_t xcd_group;
   int64_t total_blocks;
-  int32_t slice_lo, slice_hi;  // slices [slice_lo, slice_hi) of each ray
 };
 
 // Narrow [lo, hi] (slice indices, initially [0, n_a - 1]) to the slices i
@@ -193,8 +190,8 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
       ihi = max(ihi, min(P.na - 1, static_cast<int32_t>(ceilf(hi)) + 1));
     }
   }
-  ilo = max(kk_wave_min(ilo), A.slice_lo);
-  ihi = min(kk_wave_max(ihi), A.slice_hi - 1);
+  ilo = kk_wave_min(ilo);
+  ihi = kk_wave_max(ihi);
 
   float acc[R];
 #pragma unroll
@@ -295,8 +292,6 @@ int make_fp_args(const kk::FpSpec& S, int32_t rpt, FpArgs& A) {
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_fp: grid too large");
   A.xcd_group = (A.col_tiles % kk::kNumXcd == 0) ? A.col_tiles / kk::kNumXcd
                                                   : 1;
-  A.slice_lo = 0;
-  A.slice_hi = 0x7fffffff;
   return KK_OK;
 }
 
@@ -509,28 +504,12 @@ int launch_fp_spec(const kk::FpSpec& S, int32_t rows_per_thread,
   const int rc = make_fp_args(S, rpt, A);
   if (rc != KK_OK) return rc;
   const dim3 grid(static_cast<unsigned>(A.total_blocks));
-  // slice chunks launched one after another (first writes, later ones
-  // add): the waves in flight then share one slab of the texture
-  int32_t chunk = 0;
-  if (const char* env = std::getenv("KK_FP_SLICE_CHUNK")) chunk = std::atoi(env);
-  int32_t n_slices = 0;
-  for (int t = 0; t < 3; ++t)
-    n_slices = std::max(n_slices, S.plan->fp_shape[t]);
-  if (chunk <= 0 || chunk >= n_slices || A.mode == 2 || A.out_dual != 0)
-    chunk = n_slices;
-  for (int32_t lo = 0; lo < n_slices; lo += chunk) {
-    FpArgs C = A;
-    C.slice_lo = lo;
-    C.slice_hi = lo + chunk;
-    if (lo > 0) C.mode = 1;
-    if (lo == 0 && chunk == n_slices) C.slice_hi = 0x7fffffff;
-    if (S.plan->fp_all_colc)
-      launch_single<true>(rpt, grid, stream, S.plan->fp, C);
-    else
-      launch_single<false>(rpt, grid, stream, S.plan->fp, C);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_kernel");
-  }
+  if (S.plan->fp_all_colc)
+    launch_single<true>(rpt, grid, stream, S.plan->fp, A);
+  else
+    launch_single<false>(rpt, grid, stream, S.plan->fp, A);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_kernel");
   return KK_OK;
 }
 
