@@ -18,10 +18,12 @@
 //    wavefront instead of 64.
 // 2. cone_fp_t_kernel: a gather, so that no atomics are needed (an LDS
 //    float atomic add costs ~190 cycles per wavefront instruction on this
-//    part, against ~8 for a plain LDS read-modify-write).  A lane owns kJB
-//    voxels along b at one (slice, c) position, the 64 lanes of a
-//    wavefront consecutive c positions, accumulated in registers over all
-//    angles of the launch's ray axis and written once.  Per angle:
+//    part, against ~8 for a plain LDS read-modify-write).  A lane owns kJB x kJC
+//    voxels (kJB along b, kJC consecutive along c) of one slice, the 64
+//    lanes of a wavefront consecutive c pairs, accumulated in registers
+//    over all angles of the launch's ray axis and written once.  Owning
+//    two c voxels, a lane evaluates each of its samples once for both
+//    (a sample has two c taps): 3/2 evaluations per sample instead of 2.  Per angle:
 //    * circular orbits ("column-coherent" angles: the row step has no a/b
 //      part): the b coordinate of a sample depends on the detector column
 //      only, so the columns reaching the wavefront's b-run are a
@@ -43,6 +45,7 @@
 namespace {
 
 constexpr int kJB = 8;       // voxels along b per lane
+constexpr int kJC = 2;       // consecutive voxels along c per lane
 constexpr int kWaves = 4;    // wavefronts per workgroup (b-runs)
 constexpr int kThreads = 64 * kWaves;
 constexpr double kEps = 1e-6;
@@ -162,15 +165,19 @@ __device__ __forceinline__ double bcast(double v, int32_t l) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// acc[t0] += (1 - fb) s, acc[t0 + 1] += fb s for a wave-uniform t0 in
-// [-1, kJB): scalar branches, two FMAs.
-__device__ __forceinline__ void add_b_taps(float (&acc)[kJB], int32_t t0,
-                                           float fb, float s) {
+// acc[t0][cc] += (1 - fb) s[cc], acc[t0 + 1][cc] += fb s[cc] for a
+// wave-uniform t0 in [-1, kJB): scalar branches, 2 kJC FMAs.
+__device__ __forceinline__ void add_b_taps(float (&acc)[kJB][kJC],
+                                           int32_t t0, float fb,
+                                           const float (&s)[kJC]) {
 #pragma unroll
   for (int t = -1; t < kJB; ++t) {
     if (t == t0) {
-      if (t >= 0) acc[t] = fmaf(1.f - fb, s, acc[t]);
-      if (t + 1 < kJB) acc[t + 1] = fmaf(fb, s, acc[t + 1]);
+#pragma unroll
+      for (int cc = 0; cc < kJC; ++cc) {
+        if (t >= 0) acc[t][cc] = fmaf(1.f - fb, s[cc], acc[t][cc]);
+        if (t + 1 < kJB) acc[t + 1][cc] = fmaf(fb, s[cc], acc[t + 1][cc]);
+      }
     }
   }
 }
@@ -183,7 +190,7 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
   const int32_t tb = static_cast<int32_t>(bid % A.tiles_b);
   const int32_t i = static_cast<int32_t>(bid / A.tiles_b);
   const int32_t lane = threadIdx.x & 63;
-  const int32_t k = tc * 64 + lane;
+  const int32_t k = (tc * 64 + lane) * kJC;  // first of the lane's c voxels
   const int32_t jw = (tb * kWaves + static_cast<int32_t>(threadIdx.x >> 6)) *
                      kJB;
   if (jw >= A.nb) return;  // no barriers below: idle wavefronts may leave
@@ -194,16 +201,20 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
   const double kd = static_cast<double>(k);
   const float kf = static_cast<float>(k);
   const int64_t angle_elems = static_cast<int64_t>(A.rows) * A.cols;
-  float acc[kJB];
+  float acc[kJB][kJC];
 #pragma unroll
-  for (int t = 0; t < kJB; ++t) acc[t] = 0.f;
+  for (int t = 0; t < kJB; ++t)
+#pragma unroll
+    for (int cc = 0; cc < kJC; ++cc) acc[t][cc] = 0.f;
   // general path: b-tap sums of this lane at t = -1 .. kJB (row t + 1),
   // indexed per lane; every lane owns its words, so no atomics and no
   // cross-lane ordering are needed
-  __shared__ float gacc[kWaves][kJB + 2][64];
-  float(*g)[64] = gacc[threadIdx.x >> 6];
+  __shared__ float gacc[kWaves][kJB + 2][kJC][64];
+  float(*g)[kJC][64] = gacc[threadIdx.x >> 6];
 #pragma unroll
-  for (int t = 0; t < kJB + 2; ++t) g[t][lane] = 0.f;
+  for (int t = 0; t < kJB + 2; ++t)
+#pragma unroll
+    for (int cc = 0; cc < kJC; ++cc) g[t][cc][lane] = 0.f;
 
   for (int32_t q = 0; q < A.n_angles; ++q) {
     const int32_t ang = A.angles[q];
@@ -254,18 +265,20 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
           if (tm < -1 || tm >= kJB) continue;
           const double invm = bcast(inv, m), D0cm = bcast(D0c, m);
           const float iam = bcast(ia, m);
-          // rows whose c coordinate lies in [k, k + 2), found in float32
-          // and widened by 1e-2 row (float32 error ~1e-4 row); the
+          // rows whose c coordinate lies in [k, k + kJC + 1), found in
+          // float32 and widened by 1e-2 row (float32 error ~1e-4 row); the
           // samples themselves are evaluated exactly below
           int32_t r_lo = 0, r_hi = A.rows - 1;
           if (iam != 0.f) {
             const float r1 = (kf - bcast(beta, m)) * iam;
-            const float r2 = fmaf(2.f, iam, r1);
+            const float r2 = fmaf(static_cast<float>(kJC + 1), iam, r1);
             int_range(fminf(r1, r2) - 1e-2f, fmaxf(r1, r2) + 1e-2f, A.rows,
                       r_lo, r_hi);
           }
           const float* ycol = img + (cb + m) * A.rows;
-          float colsum = 0.f;
+          float colsum[kJC];
+#pragma unroll
+          for (int cc = 0; cc < kJC; ++cc) colsum[cc] = 0.f;
           // rows in groups of kRU, loads first (a lane's rows are
           // contiguous: one or two groups), summed in row order
           for (int32_t r = r_lo; r <= r_hi; r += kRU) {
@@ -282,8 +295,12 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
               kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
               // rays the forward kernel skips have yw = 0 (and clamped,
               // finite coordinates here)
-              if (r + u <= r_hi)
-                colsum = fmaf(y[u], tap_weight(ic, fcw, k), colsum);
+              if (r + u <= r_hi) {
+#pragma unroll
+                for (int cc = 0; cc < kJC; ++cc)
+                  colsum[cc] = fmaf(y[u], tap_weight(ic, fcw, k + cc),
+                                    colsum[cc]);
+              }
             }
           }
           add_b_taps(acc, tm, bcast(fb, m), colsum);
@@ -298,7 +315,7 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
       for (int corner = 0; corner < 4; ++corner) {
         const double xb = (corner & 1 ? jw + kJB + 1 + kEps : jw - kEps) -
                           P.taub;
-        const double xc = (corner & 2 ? kd + 2.0 + kEps : kd - kEps) -
+        const double xc = (corner & 2 ? kd + (kJC + 1) + kEps : kd - kEps) -
                           P.tauc;
         double pc, pr;
         ok = ok && project(P, arel, xb, xc, pc, pr);
@@ -320,33 +337,46 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
           uint32_t ic;
           float fcw;
           kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
-          const float wk = tap_weight(ic, fcw, k);
-          if (wk == 0.f) continue;
+          float wk[kJC];
+          bool any = false;
+#pragma unroll
+          for (int cc = 0; cc < kJC; ++cc) {
+            wk[cc] = tap_weight(ic, fcw, k + cc);
+            any = any || wk[cc] != 0.f;
+          }
+          if (!any) continue;
           uint32_t ib;
           float fb;
           kk_fp_coord(a1, b1, fi, nb1, ib, fb);
           const int32_t t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
           if (t0 < -1 || t0 >= kJB) continue;
-          const float val = ycol[r] * wk;
-          g[t0 + 1][lane] = fmaf(1.f - fb, val, g[t0 + 1][lane]);
-          g[t0 + 2][lane] = fmaf(fb, val, g[t0 + 2][lane]);
+          const float yv = ycol[r];
+#pragma unroll
+          for (int cc = 0; cc < kJC; ++cc) {
+            const float val = yv * wk[cc];
+            g[t0 + 1][cc][lane] = fmaf(1.f - fb, val, g[t0 + 1][cc][lane]);
+            g[t0 + 2][cc][lane] = fmaf(fb, val, g[t0 + 2][cc][lane]);
+          }
         }
       }
     }
   }
 
 #pragma unroll
-  for (int t = 0; t < kJB; ++t) acc[t] += g[t + 1][lane];
-  if (k >= A.nc) return;
+  for (int cc = 0; cc < kJC; ++cc) {
+    const int32_t kc = k + cc;
+    if (kc >= A.nc) break;
 #pragma unroll
-  for (int t = 0; t < kJB; ++t) {
-    const int32_t j = jw + t;
-    if (j >= A.nb) break;
-    const int64_t o = i * A.sa + j * A.sb + k * A.sc;
-    if (A.mode == 1)
-      A.vol[o] += acc[t];
-    else
-      A.vol[o] = acc[t];
+    for (int t = 0; t < kJB; ++t) {
+      const int32_t j = jw + t;
+      if (j >= A.nb) break;
+      const float v = acc[t][cc] + g[t + 1][cc][lane];
+      const int64_t o = i * A.sa + j * A.sb + kc * A.sc;
+      if (A.mode == 1)
+        A.vol[o] += v;
+      else
+        A.vol[o] = v;
+    }
   }
 }
 
@@ -404,7 +434,7 @@ extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
     A.nb = vol_shape[b];
     A.nc = vol_shape[c];
     A.tiles_b = (A.nb + kJB * kWaves - 1) / (kJB * kWaves);
-    A.tiles_c = (A.nc + 63) / 64;
+    A.tiles_c = (A.nc + 64 * kJC - 1) / (64 * kJC);
     const int64_t blocks = int64_t(A.na) * A.tiles_b * A.tiles_c;
     if (blocks >= (int64_t(1) << 31))
       return kk::fail(KK_ERR_UNSUPPORTED, "kk_cone_fp_transpose: grid");

@@ -62,13 +62,12 @@ def fp_samples(pgs, vg) -> int:
     return int(rays * shape[axes].sum())
 
 
-def cpu_baseline(pgs, vg, vol):
-    """FP64 oracle on one core over a bounded sample of the workload."""
+def _cpu_sample_geometry(pgs, vg):
+    """The bounded CPU sample: the middle half of angle 0's detector rows
+    (FP) and the middle half-slab of the volume along x (BP)."""
     import numpy as np
-    from threadpoolctl import threadpool_limits
 
     import kernelkit as kk
-    import oracle
     from kernelkit.geom.proj import Detector, ProjectionGeometry
 
     g = pgs[0]
@@ -91,9 +90,58 @@ def cpu_baseline(pgs, vg, vol):
     smin[0] = emin[0] + x0 * vs[0]
     smax[0] = smin[0] + slab * vs[0]
     vg_slab = kk.VolumeGeometry((slab,) + tuple(vg.shape[1:]), vs, smin, smax)
+    return g, g_band, vg_slab, rows, cols, band, r0, nx, slab
+
+
+def _cpu_task(args):
+    """One host process of the baseline: the FP64 oracle on one core, FP
+    of the volume (shared memory) into a detector band, or BP of a band
+    into a slab.  Returns the seconds it took."""
+    import numpy as np
+    from multiprocessing import shared_memory
+    from threadpoolctl import threadpool_limits
+
+    import oracle
+
+    kind, shm_name, shape, geom, vg, arg = args
+    with threadpool_limits(limits=1):
+        if kind == "fp":
+            shm = shared_memory.SharedMemory(name=shm_name)
+            try:
+                vol = np.ndarray(shape, dtype=np.float64, buffer=shm.buf)
+                t0 = time.perf_counter()
+                oracle.cone_fp(vol, [geom], vg)
+                dt = time.perf_counter() - t0
+                del vol
+            finally:
+                shm.close()
+        else:
+            t0 = time.perf_counter()
+            oracle.cone_bp(arg, [geom], vg)
+            dt = time.perf_counter() - t0
+    return dt
+
+
+def cpu_baseline(pgs, vg, vol, workers=16):
+    """FP64 oracle over a bounded sample of the workload: on one core, and
+    on ``workers`` host processes (one core each) running the same sample
+    side by side (FP and BP are independent per angle, so this is how the
+    oracle scales over cores)."""
+    import multiprocessing as mp
+    from concurrent.futures import ProcessPoolExecutor
+    from multiprocessing import shared_memory
+
+    import numpy as np
+    from threadpoolctl import threadpool_limits
+
+    import oracle
+
+    g, g_band, vg_slab, rows, cols, band, r0, nx, slab = \
+        _cpu_sample_geometry(pgs, vg)
+    vol64 = vol.astype(np.float64)
     with threadpool_limits(limits=1):
         t0 = time.perf_counter()
-        y = oracle.cone_fp(vol.astype(np.float64), [g_band], vg)
+        y = oracle.cone_fp(vol64, [g_band], vg)
         t_fp = time.perf_counter() - t0
         proj = np.zeros((1, rows, cols))
         proj[0, r0:r0 + band] = y[0]
@@ -105,14 +153,68 @@ def cpu_baseline(pgs, vg, vol):
     # in voxels
     t_fp_angle = t_fp * rows / band
     t_bp_angle = t_bp * nx / slab
-    gups = 2 * n_vox / (t_fp_angle + t_bp_angle) / 1e9
+    gups1 = 2 * n_vox / (t_fp_angle + t_bp_angle) / 1e9
+    single = {
+        "value": round(gups1, 6), "cores": 1,
+        "sample": (f"FP over {band} of {rows} detector rows "
+                   f"({t_fp:.1f} s), BP over a {slab}-slice x-slab of the "
+                   f"{nx}^3 volume ({t_bp:.1f} s)")}
+    workers = max(1, min(workers, os.cpu_count() or 1))
+    shm = shared_memory.SharedMemory(create=True, size=vol64.nbytes)
+    try:
+        np.ndarray(vol64.shape, np.float64, buffer=shm.buf)[...] = vol64
+        # a worker that dies raises BrokenProcessPool here (no hang)
+        with ProcessPoolExecutor(workers, mp_context=mp.get_context(
+                "spawn")) as pool:
+            list(pool.map(_cpu_task, [("bp", None, None, g, vg_slab,
+                                       proj[:, :1, :1])] * workers,
+                          timeout=120))   # start the workers
+            t0 = time.perf_counter()
+            list(pool.map(_cpu_task, [("fp", shm.name, vol64.shape, g_band,
+                                       vg, None)] * workers, timeout=600))
+            w_fp = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            list(pool.map(_cpu_task, [("bp", None, None, g, vg_slab, proj)]
+                          * workers, timeout=600))
+            w_bp = time.perf_counter() - t0
+    finally:
+        shm.close()
+        shm.unlink()
+    # `workers` samples in each wall time
+    t_angle = (w_fp * rows / band + w_bp * nx / slab) / workers
+    gups = 2 * n_vox / t_angle / 1e9
     return {
-        "value": round(gups, 6), "unit": "GUPS", "cores": 1, "kind": "port",
-        "sample": (f"FP64 NumPy oracle, 1 angle: FP over {band} of {rows} "
-                   f"detector rows ({t_fp:.1f} s), BP over a {slab}-slice "
-                   f"x-slab of the {nx}^3 volume ({t_bp:.1f} s); scaled to "
-                   f"one full angle of FP+BP by ray/voxel count"),
+        "value": round(gups, 6), "unit": "GUPS", "cores": workers,
+        "kind": "port",
+        "sample": (f"FP64 NumPy oracle, {workers} host processes of one "
+                   f"core each, each running 1 angle's sample side by "
+                   f"side: FP over {band} of {rows} detector rows "
+                   f"({w_fp:.1f} s wall), BP over a {slab}-slice x-slab of "
+                   f"the {nx}^3 volume ({w_bp:.1f} s wall); scaled to full "
+                   f"angles of FP+BP by ray/voxel count"),
+        "single_core": single,
     }
+
+
+def l2_read_rate(stream, mib=2, blocks=2048, iters=1024):
+    """Aggregate L2 read rate (GB/s) from ``kk_probe_l2_read``: float4
+    streams over a ``mib`` MiB buffer, which fits each XCD's 4 MiB L2
+    but not a CU's L1."""
+    import torch
+
+    from kernelkit import _native
+    buf = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32,
+                     device="cuda")
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    lib = _native.lib()
+
+    def run():
+        _native.check(lib.kk_probe_l2_read(
+            _native._vp(buf.data_ptr()), buf.numel(), blocks, iters,
+            _native._vp(out.data_ptr()), _native.stream_ptr(stream)))
+    run()
+    ms = min(_event_ms(run, stream) for _ in range(5))
+    return blocks * 256 * 8 * iters * 16 / (ms * 1e-3) / 1e9
 
 
 def _event_ms(fn, stream, reps=1):
@@ -238,7 +340,8 @@ def algorithms_full(pgs, vg, x, iters=50):
 def autograd_diff():
     """Forward + backward of ||A x - y||^2 through the exact transpose
     against two plain calls (forward projection + backprojection) at 256^3,
-    360 angles, 384^2 detector."""
+    360 angles, 384^2 detector; also through the reference's default
+    autograd operator (voxel-driven backprojector in the backward pass)."""
     import numpy as np
     import torch
 
@@ -254,6 +357,7 @@ def autograd_diff():
     y = A(x)
     v = torch.empty_like(x)
     fpm = XrayForwardProjection(pgs, vg, matched=True)
+    fpu = XrayForwardProjection(pgs, vg)   # the reference's default
     xx = x[None, None].clone().requires_grad_(True)
     yy = (y * 1.01)[None, None]
     stream = torch.cuda.current_stream(dev)
@@ -262,13 +366,16 @@ def autograd_diff():
         A(x, out=y)
         A.T(y, out=v)
 
-    def fwd_bwd():
-        xx.grad = None
-        ((fpm(xx) - yy) ** 2).sum().backward()
+    def fwd_bwd(module):
+        def run():
+            xx.grad = None
+            ((module(xx) - yy) ** 2).sum().backward()
+        return run
 
-    plain(), fwd_bwd()
+    plain(), fwd_bwd(fpm)(), fwd_bwd(fpu)()
     t_plain = _event_ms(plain, stream, reps=3)
-    t_ag = _event_ms(fwd_bwd, stream, reps=3)
+    t_ag = _event_ms(fwd_bwd(fpm), stream, reps=3)
+    t_agu = _event_ms(fwd_bwd(fpu), stream, reps=3)
     t_t = _event_ms(lambda: A.matched_T(y, out=v), stream, reps=3)
     updates = int(np.prod(vg.shape)) * len(pgs)
     return {
@@ -276,6 +383,8 @@ def autograd_diff():
         "plain_fp_plus_bp_ms": round(t_plain, 3),
         "autograd_fwd_bwd_ms": round(t_ag, 3),
         "ratio_vs_plain": round(t_ag / t_plain, 3),
+        "unmatched_autograd_fwd_bwd_ms": round(t_agu, 3),
+        "unmatched_ratio_vs_plain": round(t_agu / t_plain, 3),
         "transpose_ms": round(t_t, 3),
         "transpose_gups": round(updates / (t_t * 1e-3) / 1e9, 1),
     }
@@ -469,6 +578,8 @@ def main(argv=None):
 
     t_ftex, t_fp, t_btex, t_bp = mean(0, 1), mean(1, 2), mean(2, 3), \
         mean(3, 4)
+    l2_rate = l2_read_rate(stream)
+    _log(f"L2 read probe: {l2_rate:.0f} GB/s")
     ms_step = wall * 1e3 / args.steps
     updates = int(np.prod(shape)) * n
     value = 2 * updates / (ms_step * 1e-3) / 1e9
@@ -518,6 +629,8 @@ def main(argv=None):
             "bytes_per_update": round(fp_bytes / updates, 4),
             "hbm_compulsory_GBs": round(
                 compulsory_fp / (t_fp * 1e-3) / 1e9, 1),
+            "l2_measured_GBs": round(l2_rate, 1),
+            "frac_of_measured_l2": round(achieved / l2_rate, 4),
             "hbm_peak": HBM_PEAK_GBS,
             "bp_kernel": {
                 "achieved": round(bp_bytes / (t_bp * 1e-3) / 1e9, 1),
@@ -549,7 +662,7 @@ def main(argv=None):
         torch.cuda.empty_cache()
         out["large"] = large_unrolled()
     if not args.no_cpu_baseline:
-        _log("cpu baseline (FP64 oracle, 1 core) ...")
+        _log("cpu baseline (FP64 oracle, 1 core and 16 processes) ...")
         out["cpu_baseline"] = cpu_baseline(pgs, vg, vol)
     print(json.dumps(out), flush=True)
     return 0

@@ -270,6 +270,18 @@ int kk_filter_rows_to_bp_texture(const kk_filter* filter, const float* in,
                                  int32_t cols, int32_t fuse_preweight,
                                  void* stream);
 
+/* ---- measurement probe -------------------------------------------- */
+
+/* No counterpart in the reference.  Streams float4 reads over buf (n_floats
+ * floats, n_floats / 4 a power of two, 16-byte aligned) with `blocks`
+ * workgroups of 256 threads, each thread reading 8 * iters float4 values
+ * (addresses wrap around the buffer); writes one float per thread to out
+ * (blocks * 256 floats).  A buffer that fits each XCD's L2 but not a CU's
+ * L1 measures the aggregate L2 read rate (bytes = blocks * 256 * 8 * iters
+ * * 16). */
+int kk_probe_l2_read(const float* buf, int64_t n_floats, int32_t blocks,
+                     int32_t iters, float* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

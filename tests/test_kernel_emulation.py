@@ -267,6 +267,7 @@ def test_full_config_fp_table_precision():
 
 
 _KJB = 8        # cone_fp_t_kernel: voxels along b per lane
+_KJC = 2        # and consecutive voxels along c
 _KEPS = 1e-6    # its widening of candidate intervals (voxels)
 
 
@@ -327,6 +328,7 @@ def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
     (R, C): the candidate enumeration of both of its paths, with the
     kernel's float64/float32 split."""
     jw = (J // _KJB) * _KJB
+    K = (K // _KJC) * _KJC          # the lane's first c voxel
     arel = float(i) + float(P["sig"])
     taub, tauc = float(P["taub"]), float(P["tauc"])
     g = {x: float(P[x]) for x in ("qa", "qb", "qc", "ua", "ub", "uc", "vc")}
@@ -350,8 +352,8 @@ def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
         with np.errstate(divide="ignore", over="ignore"):
             ia = np.where(steep, f32(1) / alpha, f32(0))
             r1 = (K.astype(f32) - beta) * ia
-            # fmaf(2, ia, r1): exact in float64, rounded once
-            r2 = (2.0 * ia.astype(np.float64) + r1).astype(f32)
+            # fmaf(kJC + 1, ia, r1): exact in float64, rounded once
+            r2 = ((_KJC + 1.0) * ia.astype(np.float64) + r1).astype(f32)
         r_lo, r_hi = _int_range(np.minimum(r1, r2) - f32(1e-2),
                                 np.maximum(r1, r2) + f32(1e-2), rows)
         r_lo, r_hi = np.where(steep, r_lo, 0), np.where(steep, r_hi, rows - 1)
@@ -359,7 +361,7 @@ def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
         cs, rs, oks = [], [], []
         for corner in range(4):
             xb = (jw + _KJB + 1 + eps if corner & 1 else jw - eps) - taub
-            xc = (K + 2.0 + eps if corner & 2 else K - eps) - tauc
+            xc = (K + _KJC + 1.0 + eps if corner & 2 else K - eps) - tauc
             pc, pr, ok = _project_corner(P, arel, xb, xc)
             cs.append(pc), rs.append(pr), oks.append(ok)
         ok = np.logical_and.reduce(oks)
