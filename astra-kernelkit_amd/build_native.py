@@ -26,8 +26,6 @@ COMMON_FLAGS = [
     "-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
     # packed FP32 instructions are disabled for all device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
-    # hardware float atomics for the scatter (transpose) kernels
-    "-munsafe-fp-atomics",
     "-Wall", "-Wno-unused-result",
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]

@@ -182,7 +182,25 @@ __device__ __forceinline__ void add_b_taps(float (&acc)[kJB][kJC],
   }
 }
 
-__global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
+// Sample of row r in a column-coherent column (D0c, inv: the column's
+// kk_fp_ray terms): adds y times its c-tap weight to the lane's kJC
+// column sums (voxels k .. k + kJC - 1).
+__device__ __forceinline__ void add_c_taps(const kk::FpAngle& P, double D0c,
+                                           double inv, double fi, double nc1,
+                                           int32_t r, int32_t k, float y,
+                                           float (&colsum)[kJC]) {
+  const double Dc = fma(static_cast<double>(r), P.vc, D0c);
+  const double a2 = Dc * inv;
+  const double b2 = fma(a2, P.sig, P.tauc);
+  uint32_t ic;
+  float fcw;
+  kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
+#pragma unroll
+  for (int cc = 0; cc < kJC; ++cc)
+    colsum[cc] = fmaf(y, tap_weight(ic, fcw, k + cc), colsum[cc]);
+}
+
+__global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
     const kk::FpAngle* __restrict__ tab, FptArgs A) {
   int64_t bid = blockIdx.x;
   const int32_t tc = static_cast<int32_t>(bid % A.tiles_c);
@@ -279,30 +297,21 @@ __global__ __launch_bounds__(kThreads) void cone_fp_t_kernel(
           float colsum[kJC];
 #pragma unroll
           for (int cc = 0; cc < kJC; ++cc) colsum[cc] = 0.f;
-          // rows in groups of kRU, loads first (a lane's rows are
-          // contiguous: one or two groups), summed in row order
-          for (int32_t r = r_lo; r <= r_hi; r += kRU) {
+          // rows in groups of kRU, loads first, then one at a time (a
+          // lane's rows are contiguous, typically kRU + 1 of them), summed
+          // in row order; rays the forward kernel skips have yw = 0 (and
+          // clamped, finite coordinates here)
+          int32_t r = r_lo;
+          for (; r + (kRU - 1) <= r_hi; r += kRU) {
             float y[kRU];
 #pragma unroll
-            for (int u = 0; u < kRU; ++u) y[u] = ycol[min(r + u, r_hi)];
+            for (int u = 0; u < kRU; ++u) y[u] = ycol[r + u];
 #pragma unroll
-            for (int u = 0; u < kRU; ++u) {
-              const double Dc = fma(static_cast<double>(r + u), P.vc, D0cm);
-              const double a2 = Dc * invm;
-              const double b2 = fma(a2, P.sig, P.tauc);
-              uint32_t ic;
-              float fcw;
-              kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
-              // rays the forward kernel skips have yw = 0 (and clamped,
-              // finite coordinates here)
-              if (r + u <= r_hi) {
-#pragma unroll
-                for (int cc = 0; cc < kJC; ++cc)
-                  colsum[cc] = fmaf(y[u], tap_weight(ic, fcw, k + cc),
-                                    colsum[cc]);
-              }
-            }
+            for (int u = 0; u < kRU; ++u)
+              add_c_taps(P, D0cm, invm, fi, nc1, r + u, k, y[u], colsum);
           }
+          for (; r <= r_hi; ++r)
+            add_c_taps(P, D0cm, invm, fi, nc1, r, k, ycol[r], colsum);
           add_b_taps(acc, tm, bcast(fb, m), colsum);
         }
       }
