@@ -84,6 +84,24 @@ __device__ __forceinline__ void split_clamped(float off, int32_t base,
   idx = min(max(i, 0), hi);
 }
 
+// split_clamped for a coordinate known to lie in [0, hi): no clamps, the
+// same index and weight.
+__device__ __forceinline__ void split_inside(float off, int32_t base,
+                                             int32_t& idx, float& frac) {
+  const float fl = floorf(off);
+  idx = base + static_cast<int32_t>(fl);
+  frac = off - fl;
+}
+
+// Whether base + [lo, hi_off] lies in [1, hi - 1]: a run of coordinates
+// there never reaches split_clamped's clamps (one texel of margin covers
+// the float32 evaluation of this test).
+__device__ __forceinline__ bool run_inside(int32_t base, float lo,
+                                           float hi_off, int32_t hi) {
+  const float b = static_cast<float>(base);
+  return b + lo >= 1.f && b + hi_off <= static_cast<float>(hi - 1);
+}
+
 // One workgroup's tile (16 x 16 columns of K voxels) of logical block id
 // `bid`.
 template <int K>
@@ -140,14 +158,31 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
       const float fv0 = static_cast<float>(vc0 - vfl);
       uint32_t off[K];
       float fu[K], fv[K];
+      const float ku = static_cast<float>(K - 1) * gu;
+      const float kv = static_cast<float>(K - 1) * gv;
+      const bool inside =
+          run_inside(iub, fu0 + fminf(ku, 0.f), fu0 + fmaxf(ku, 0.f), cmax) &&
+          run_inside(ivb, fv0 + fminf(kv, 0.f), fv0 + fmaxf(kv, 0.f), rmax);
+      if (__all(inside)) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const float fk = static_cast<float>(k);
-        int32_t iu, iv;
-        split_clamped(fmaf(fk, gu, fu0), iub, cmax, iu, fu[k]);
-        split_clamped(fmaf(fk, gv, fv0), ivb, rmax, iv, fv[k]);
-        off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
-                 (static_cast<uint32_t>(iu) << 3);
+        for (int k = 0; k < K; ++k) {
+          const float fk = static_cast<float>(k);
+          int32_t iu, iv;
+          split_inside(fmaf(fk, gu, fu0), iub, iu, fu[k]);
+          split_inside(fmaf(fk, gv, fv0), ivb, iv, fv[k]);
+          off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
+                   (static_cast<uint32_t>(iu) << 3);
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const float fk = static_cast<float>(k);
+          int32_t iu, iv;
+          split_clamped(fmaf(fk, gu, fu0), iub, cmax, iu, fu[k]);
+          split_clamped(fmaf(fk, gv, fv0), ivb, rmax, iv, fv[k]);
+          off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
+                   (static_cast<uint32_t>(iu) << 3);
+        }
       }
       kk_float4u tp[K];
 #pragma unroll
@@ -213,19 +248,48 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
       const int32_t ivb = static_cast<int32_t>(vb);
       uint32_t off[K];
       float fu[K], fv[K], w[K];
+      // the run's end points: with den of one sign along the run, u and v
+      // are monotonic in k (ratios of affine forms), so end points inside
+      // the detector put the whole run inside (non-finite values fail the
+      // comparisons)
+      const float kl = static_cast<float>(K - 1);
+      const float dK = fmaf(kl, dzf, d0f);
+      const float rf0 = __builtin_amdgcn_rcpf(d0f);
+      const float rfK = __builtin_amdgcn_rcpf(dK);
+      const float du0 = e0u * rf0, duK = fmaf(kl, e1u, e0u) * rfK;
+      const float dv0 = e0v * rf0, dvK = fmaf(kl, e1v, e0v) * rfK;
+      const bool inside = d0f * dK > 0.f &&
+                          run_inside(iub, fminf(du0, duK), fmaxf(du0, duK),
+                                     cmax) &&
+                          run_inside(ivb, fminf(dv0, dvK), fmaxf(dv0, dvK),
+                                     rmax);
+      if (__all(inside)) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
-        const float fk = static_cast<float>(k);
-        const float r = __builtin_amdgcn_rcpf(fmaf(fk, dzf, d0f));
-        const float du = fmaf(fk, e1u, e0u) * r;
-        const float dv = fmaf(fk, e1v, e0v) * r;
-        const bool ok = isfinite(du) && isfinite(dv);
-        int32_t iu, iv;
-        split_clamped(ok ? du : 0.f, ok ? iub : 0, cmax, iu, fu[k]);
-        split_clamped(ok ? dv : 0.f, ok ? ivb : 0, rmax, iv, fv[k]);
-        off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
-                 (static_cast<uint32_t>(iu) << 3);
-        w[k] = ok ? r * r : 0.f;
+        for (int k = 0; k < K; ++k) {
+          const float fk = static_cast<float>(k);
+          const float r = __builtin_amdgcn_rcpf(fmaf(fk, dzf, d0f));
+          int32_t iu, iv;
+          split_inside(fmaf(fk, e1u, e0u) * r, iub, iu, fu[k]);
+          split_inside(fmaf(fk, e1v, e0v) * r, ivb, iv, fv[k]);
+          off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
+                   (static_cast<uint32_t>(iu) << 3);
+          w[k] = r * r;
+        }
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const float fk = static_cast<float>(k);
+          const float r = __builtin_amdgcn_rcpf(fmaf(fk, dzf, d0f));
+          const float du = fmaf(fk, e1u, e0u) * r;
+          const float dv = fmaf(fk, e1v, e0v) * r;
+          const bool ok = isfinite(du) && isfinite(dv);
+          int32_t iu, iv;
+          split_clamped(ok ? du : 0.f, ok ? iub : 0, cmax, iu, fu[k]);
+          split_clamped(ok ? dv : 0.f, ok ? ivb : 0, rmax, iv, fv[k]);
+          off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
+                   (static_cast<uint32_t>(iu) << 3);
+          w[k] = ok ? r * r : 0.f;
+        }
       }
       kk_float4u tp[K];
 #pragma unroll
@@ -263,8 +327,13 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
   }
 }
 
+// Register caps without spills: 5 wavefronts per SIMD up to 8 voxels per
+// thread, 3 at 16 (the unclamped and clamped branches together would
+// otherwise take 102 and 176 VGPRs, i.e. 4 and 2 waves).
+constexpr int bp_min_blocks(int k) { return k <= 8 ? 5 : 3; }
+
 template <int K>
-__global__ __launch_bounds__(256) void cone_bp_kernel(
+__global__ __launch_bounds__(256, bp_min_blocks(K)) void cone_bp_kernel(
     const kk::BpAngle* __restrict__ tab, BpArgs A) {
   bp_tile<K>(tab, A, kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
 }
@@ -276,7 +345,7 @@ struct BpItem {
 
 // Many problems, one persistent launch (see cone_fp_batch_kernel).
 template <int K>
-__global__ __launch_bounds__(256) void cone_bp_batch_kernel(
+__global__ __launch_bounds__(256, bp_min_blocks(K)) void cone_bp_batch_kernel(
     const BpItem* __restrict__ items, const int64_t* __restrict__ first,
     int32_t n, int64_t total) {
   for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {

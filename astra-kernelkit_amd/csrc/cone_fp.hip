@@ -83,6 +83,27 @@ __device__ __forceinline__ void clip_range(float a, float beta, float n1,
   hi = fminf(hi, fmaxf(t0, t1));
 }
 
+// Narrow [lo, hi] to the slices i with 1 <= a*i + beta <= n1 - 1: one
+// voxel inside the clamp range [0, n1], which covers the float32 error of
+// this estimate (~1e-4 voxel).  A coordinate is linear in i, so for flat
+// slopes the end points decide.
+__device__ __forceinline__ void inner_range(float a, float beta, float n1,
+                                            float& lo, float& hi) {
+  if (fabsf(a) < 1e-3f) {
+    const float e0 = fmaf(a, lo, beta), e1 = fmaf(a, hi, beta);
+    if (!(fminf(e0, e1) >= 1.f && fmaxf(e0, e1) <= n1 - 1.f)) {
+      lo = 3e38f;
+      hi = -3e38f;
+    }
+    return;
+  }
+  const float inv = 1.f / a;
+  const float t0 = (1.f - beta) * inv;
+  const float t1 = (n1 - 1.f - beta) * inv;
+  lo = fmaxf(lo, fminf(t0, t1));
+  hi = fminf(hi, fmaxf(t0, t1));
+}
+
 // Bilinear value from the 16-byte tap record (v[b][c], v[b+1][c],
 // v[b][c+1], v[b+1][c+1]): lerp along b, then along c.
 __device__ __forceinline__ float bilerp(kk_float4u t, float fb, float fc) {
@@ -92,11 +113,10 @@ __device__ __forceinline__ float bilerp(kk_float4u t, float fb, float fc) {
 }
 
 // One tracing pass over slices [i0, i1] (wave-uniform) for R rays of a
-// thread.  COLC: a1/b1 are shared by the R rays.
-// One tracing pass over slices [i0, i1] (wave-uniform) for R rays of a
 // thread.  COLC: the b slope/intercept are shared by the R rays (nb1 = 1
-// entry in a1/b1).
-template <int R, int NB>
+// entry in a1/b1).  CLAMP = false: every coordinate of the pass is known
+// to lie inside the clamp range, which is then skipped.
+template <int R, int NB, bool CLAMP>
 __device__ __forceinline__ void trace(const kk::FpAngle& P,
                                       const char* __restrict__ frame,
                                       const double (&a1)[NB],
@@ -115,7 +135,10 @@ __device__ __forceinline__ void trace(const kk::FpAngle& P,
     uint32_t offb = 0;
     if constexpr (COLC) {
       uint32_t ib;
-      kk_fp_coord(a1[0], b1[0], fi, nb1, ib, fbs);
+      if constexpr (CLAMP)
+        kk_fp_coord(a1[0], b1[0], fi, nb1, ib, fbs);
+      else
+        kk_fp_coord_in(a1[0], b1[0], fi, ib, fbs);
       offb = __umul24(ib, sbb);
     }
 #pragma unroll
@@ -124,12 +147,18 @@ __device__ __forceinline__ void trace(const kk::FpAngle& P,
       uint32_t ob = offb;
       if constexpr (!COLC) {
         uint32_t ib;
-        kk_fp_coord(a1[k], b1[k], fi, nb1, ib, fb);
+        if constexpr (CLAMP)
+          kk_fp_coord(a1[k], b1[k], fi, nb1, ib, fb);
+        else
+          kk_fp_coord_in(a1[k], b1[k], fi, ib, fb);
         ob = __umul24(ib, sbb);
       }
       uint32_t ic;
       float fc;
-      kk_fp_coord(a2[k], b2[k], fi, nc1, ic, fc);
+      if constexpr (CLAMP)
+        kk_fp_coord(a2[k], b2[k], fi, nc1, ic, fc);
+      else
+        kk_fp_coord_in(a2[k], b2[k], fi, ic, fc);
       const uint32_t off = ob + (ic << 3);
       acc[k] += bilerp(kk_ld4(slice, off), fb, fc);
     }
@@ -160,52 +189,73 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
   double a1[NB], b1[NB], a2[R], b2[R];
   float wscale[R];
   int32_t ilo = 0x7fffffff, ihi = -1;
+  float jl = 0.f, jh = last;  // this thread's rays all inside
 #pragma unroll
   for (int k = 0; k < R; ++k) {
     const int32_t row = row0 + kWave * k;
     // rays outside the detector, or parallel to the slices, contribute
-    // nothing: park them in the halo (for column-coherent angles the b
-    // part of every ray of the thread equals that of its first ray)
+    // nothing: they are parked at padded coordinate 1 (inside every
+    // range, so also valid for unclamped passes) with weight 0 (for
+    // column-coherent angles the b part of every ray of the thread equals
+    // that of its first ray)
     double ra1, rb1;
     const bool ok = row < A.rows && col < A.cols &&
                     kk_fp_ray(P, row, col, ra1, rb1, a2[k], b2[k]);
     if (k < NB) {
       a1[k] = ok ? ra1 : 0.0;
-      b1[k] = ok ? rb1 : -1.0;
+      b1[k] = ok ? rb1 : 1.0;
     }
     wscale[k] = 0.f;
     if (!ok) {
       a2[k] = 0.0;
-      b2[k] = -1.0;
+      b2[k] = 1.0;
       continue;
     }
     wscale[k] = kk_fp_weight(P, ra1, a2[k]);
+    const float fa1 = static_cast<float>(ra1), fb1 = static_cast<float>(rb1);
+    const float fa2 = static_cast<float>(a2[k]);
+    const float fb2 = static_cast<float>(b2[k]);
     float lo = 0.f, hi = last;
-    clip_range(static_cast<float>(ra1), static_cast<float>(rb1), nb1, last,
-               lo, hi);
-    clip_range(static_cast<float>(a2[k]), static_cast<float>(b2[k]), nc1,
-               last, lo, hi);
+    clip_range(fa1, fb1, nb1, last, lo, hi);
+    clip_range(fa2, fb2, nc1, last, lo, hi);
     if (lo <= hi + 2.f) {
       ilo = min(ilo, max(0, static_cast<int32_t>(floorf(lo)) - 1));
       ihi = max(ihi, min(P.na - 1, static_cast<int32_t>(ceilf(hi)) + 1));
     }
+    inner_range(fa1, fb1, nb1, jl, jh);
+    inner_range(fa2, fb2, nc1, jl, jh);
   }
   ilo = kk_wave_min(ilo);
   ihi = kk_wave_max(ihi);
+  // slices where every ray of the wave is inside: no clamps there
+  int32_t jlo = 1, jhi = 0;
+  if (jl <= jh) {
+    jlo = max(0, static_cast<int32_t>(ceilf(jl)));
+    jhi = min(P.na - 1, static_cast<int32_t>(floorf(jh)));
+  }
+  jlo = max(kk_wave_max(jlo), ilo);
+  jhi = min(kk_wave_min(jhi), ihi);
 
   float acc[R];
 #pragma unroll
   for (int k = 0; k < R; ++k) acc[k] = 0.f;
   const char* frame = A.tex + P.tex_off;
   if (ilo <= ihi) {
+    // clamped head, unclamped interior, clamped tail (or one clamped pass)
+    const bool split = jlo <= jhi;
+    const int32_t h1 = split ? jlo - 1 : ihi;
     if constexpr (ALL_COLC) {
-      trace<R, 1>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
+      trace<R, 1, true>(P, frame, a1, b1, a2, b2, ilo, h1, acc);
+      if (split) {
+        trace<R, 1, false>(P, frame, a1, b1, a2, b2, jlo, jhi, acc);
+        trace<R, 1, true>(P, frame, a1, b1, a2, b2, jhi + 1, ihi, acc);
+      }
     } else {
       if (P.colc) {
         const double a1c[1] = {a1[0]}, b1c[1] = {b1[0]};
-        trace<R, 1>(P, frame, a1c, b1c, a2, b2, ilo, ihi, acc);
+        trace<R, 1, true>(P, frame, a1c, b1c, a2, b2, ilo, ihi, acc);
       } else {
-        trace<R, R>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
+        trace<R, R, true>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
       }
     }
   }

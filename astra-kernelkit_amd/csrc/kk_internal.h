@@ -266,3 +266,13 @@ __device__ __forceinline__ void kk_fp_coord(double a, double b, double fi,
   frac = static_cast<float>(x - fl);
 }
 
+// kk_fp_coord for a coordinate known to lie in [0, n1]: the same index and
+// weight without the clamp.
+__device__ __forceinline__ void kk_fp_coord_in(double a, double b, double fi,
+                                               uint32_t& idx, float& frac) {
+  const double x = fma(a, fi, b);
+  const double fl = floor(x);
+  idx = static_cast<uint32_t>(fl);
+  frac = static_cast<float>(x - fl);
+}
+

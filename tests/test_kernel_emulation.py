@@ -419,3 +419,77 @@ def test_fp_transpose_emulation(pert):
     # windows a quarter voxel short on each side lose taps: the coverage
     # check can fail
     assert emulate_fp_t(y, table, vg.shape, eps=-0.25)[1] > 0
+
+
+def _inner_range(a, beta, n1, lo, hi):
+    """cone_fp.hip inner_range() in float32, vectorised over rays."""
+    a, beta = a.astype(f32), beta.astype(f32)
+    n1 = f32(n1)
+    flat = np.abs(a) < f32(1e-3)
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        e0 = (a * lo + beta).astype(f32)
+        e1 = (a * hi + beta).astype(f32)
+        ok_flat = (np.minimum(e0, e1) >= 1) & (np.maximum(e0, e1) <= n1 - 1)
+        inv = (f32(1) / a).astype(f32)
+        t0 = ((f32(1) - beta) * inv).astype(f32)
+        t1 = ((n1 - f32(1) - beta) * inv).astype(f32)
+    lo2 = np.where(flat, np.where(ok_flat, lo, f32(3e38)),
+                   np.maximum(lo, np.minimum(t0, t1)))
+    hi2 = np.where(flat, np.where(ok_flat, hi, f32(-3e38)),
+                   np.minimum(hi, np.maximum(t0, t1)))
+    return lo2.astype(f32), hi2.astype(f32)
+
+
+@pytest.mark.parametrize("name,angles", [
+    ("full", (0, 45, 90, 137, 300, 719)), ("diff", (0, 13, 200)),
+    ("large", (0, 611, 1000)), ("zmajor", (0, 1, 2, 3))])
+def test_fp_unclamped_interior_is_inside(name, angles):
+    """The forward kernel traces the slices where all of a wave's rays are
+    inside the volume without clamping.  Their range comes from a float32
+    estimate (inner_range) with one voxel of margin; here every slice it
+    admits has both exact coordinates at least half a voxel inside the
+    clamp range [0, n + 1], so the unclamped loads stay in bounds."""
+    if name == "zmajor":
+        g0 = kk.rotate(workloads.circular_geometry(
+            1, sod=120.0, odd=60.0, rows=24, cols=28, pixel=1.2)[0],
+            pitch=np.pi / 2)
+        pgs = [kk.rotate(g0, yaw=t) for t in
+               np.linspace(0, 2 * np.pi, 4, False)]
+        vg = kk.resolve_volume_geometry(shape=(16, 20, 12), voxel_size=1.0)
+    else:
+        pgs, vg = workloads.config_geometry(name)
+    sub = [pgs[k] for k in angles]
+    t = _fp_table(sub, vg)
+    rows, cols = int(sub[0].detector.rows), int(sub[0].detector.cols)
+    R, C = np.meshgrid(np.arange(rows, dtype=np.float64),
+                       np.arange(cols, dtype=np.float64), indexing="ij")
+    checked = 0
+    for P in t:
+        na, nb, nc = int(P["na"]), int(P["nb"]), int(P["nc"])
+        a1, b1, a2, b2 = (v.ravel() for v in _fp_rays(P, R, C))
+        ok = (np.isfinite(a1) & np.isfinite(b1) & np.isfinite(a2)
+              & np.isfinite(b2))
+        a1, b1, a2, b2 = a1[ok], b1[ok], a2[ok], b2[ok]
+        lo = np.zeros(a1.shape, f32)
+        hi = np.full(a1.shape, f32(na - 1))
+        lo, hi = _inner_range(a1, b1, nb + 1, lo, hi)
+        lo, hi = _inner_range(a2, b2, nc + 1, lo, hi)
+        with np.errstate(invalid="ignore", over="ignore"):
+            jlo = np.maximum(0, np.ceil(lo).clip(-1, na).astype(np.int64))
+            jhi = np.minimum(na - 1,
+                             np.floor(hi).clip(-1, na).astype(np.int64))
+        for i in (0, na // 3, na // 2, na - 1):
+            m = (jlo <= i) & (i <= jhi)
+            xb = a1[m] * i + b1[m]
+            xc = a2[m] * i + b2[m]
+            assert np.all((xb >= 0.5) & (xb <= nb + 0.5)), (name, i)
+            assert np.all((xc >= 0.5) & (xc <= nc + 0.5)), (name, i)
+            checked += int(m.sum())
+        # the end points of each admitted range too
+        for j in (jlo, jhi):
+            m = jlo <= jhi
+            xb = a1[m] * j[m] + b1[m]
+            xc = a2[m] * j[m] + b2[m]
+            assert np.all((xb >= 0.5) & (xb <= nb + 0.5)), name
+            assert np.all((xc >= 0.5) & (xc <= nc + 0.5)), name
+    assert checked > 0
