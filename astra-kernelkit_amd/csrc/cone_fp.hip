@@ -255,7 +255,11 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
         const double a1c[1] = {a1[0]}, b1c[1] = {b1[0]};
         trace<R, 1, true>(P, frame, a1c, b1c, a2, b2, ilo, ihi, acc);
       } else {
-        trace<R, R, true>(P, frame, a1, b1, a2, b2, ilo, ihi, acc);
+        trace<R, R, true>(P, frame, a1, b1, a2, b2, ilo, h1, acc);
+        if (split) {
+          trace<R, R, false>(P, frame, a1, b1, a2, b2, jlo, jhi, acc);
+          trace<R, R, true>(P, frame, a1, b1, a2, b2, jhi + 1, ihi, acc);
+        }
       }
     }
   }
@@ -280,8 +284,14 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
   }
 }
 
+// Register caps: 8 wavefronts per SIMD up to two rows per thread, 6 at
+// four (5 in the batch kernel), whose float64 ray data would otherwise
+// spill.
+constexpr int fp_min_blocks(int r) { return r <= 2 ? 8 : 6; }
+constexpr int fp_batch_min_blocks(int r) { return r <= 2 ? 8 : 5; }
+
 template <int R, bool ALL_COLC>
-__global__ __launch_bounds__(256, 8) void cone_fp_kernel(
+__global__ __launch_bounds__(256, fp_min_blocks(R)) void cone_fp_kernel(
     const kk::FpAngle* __restrict__ tab, FpArgs A) {
   fp_tile<R, ALL_COLC>(tab, A,
                        kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
@@ -354,7 +364,7 @@ struct FpItem {
 };
 
 template <int R, bool ALL_COLC>
-__global__ __launch_bounds__(256, 8) void cone_fp_batch_kernel(
+__global__ __launch_bounds__(256, fp_batch_min_blocks(R)) void cone_fp_batch_kernel(
     const FpItem* __restrict__ items, const int64_t* __restrict__ first,
     int32_t n, int64_t total) {
   for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {

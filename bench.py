@@ -231,7 +231,7 @@ def _event_ms(fn, stream, reps=1):
     return a.elapsed_time(b) / reps
 
 
-def batched_sirt(n_problems=64, iters=30, seed=0):
+def batched_sirt(n_problems=64, iters=30, seed=0, scale=1.0):
     """SIRT on 64 uneven problems: the whole batch per iteration in two
     persistent launches, the loop captured in a HIP graph, against the
     per-problem path (``algo.sirt``'s per-iteration launches, problem after
@@ -243,7 +243,7 @@ def batched_sirt(n_problems=64, iters=30, seed=0):
     from kernelkit.batch import ConeBatch
 
     dev = torch.device("cuda", 0)
-    problems = workloads.batch_problems(n_problems, seed=seed)
+    problems = workloads.batch_problems(n_problems, seed=seed, scale=scale)
     B = ConeBatch(problems)
     vols = [torch.from_numpy(workloads.ellipsoid_phantom(
         vg.shape, n=10, seed=seed + k)).to(dev)
@@ -271,11 +271,13 @@ def batched_sirt(n_problems=64, iters=30, seed=0):
     diff = max(float((a - b).abs().max() / b.abs().max().clamp_min(1e-30))
                for a, b in zip(st.x, [s.x for s in plain]))
     del plain, st
+    lo_v, hi_v = (max(8, int(round(s * scale))) for s in (64, 256))
+    lo_d, hi_d = int(96 * scale), int(384 * scale)
     return {
-        "workload": (f"{n_problems} problems, 64^3-256^3 (cubic and "
-                     "anisotropic), 90-360 angles, 96^2-384^2 detectors, "
-                     f"tilted/shifted cone_vec geometry; {iters} SIRT "
-                     "iterations"),
+        "workload": (f"{n_problems} problems, {lo_v}^3-{hi_v}^3 (cubic and "
+                     f"anisotropic), 90-360 angles, {lo_d}^2-{hi_d}^2 "
+                     f"detectors, tilted/shifted cone_vec geometry; {iters} "
+                     "SIRT iterations"),
         "updates_per_iteration": 2 * updates,
         "batched_graph_ms": round(t_graph, 2),
         "batched_eager_ms": round(t_eager, 2),
@@ -650,6 +652,9 @@ def main(argv=None):
     if "batched" in extras:
         _log("batched SIRT vs per-problem ...")
         out["batched"] = batched_sirt()
+        # the same generator at a quarter of the sizes: problems small
+        # enough for per-problem launches to be latency-bound
+        out["batched_small"] = batched_sirt(scale=0.25)
     if "autograd" in extras:
         _log("autograd (exact transpose) vs plain ...")
         out["autograd"] = autograd_diff()
