@@ -15,8 +15,10 @@
 //   b-neighbour) float2 pairs with a zero halo (kk_internal.h), so a sample
 //   is: one FMA per coordinate, a clamp into the halo, v_cvt_u32 + v_fract
 //   for the tap index and weight, and ONE 16-byte load for all four taps
-//   (the texture-address unit, not the ALU, limits a gather kernel: one
-//   64-lane load instruction costs it ~16 cycles whatever its width);
+//   (a 64-lane gather instruction costs the texture-address unit ~21
+//   cycles for 8 or 16 bytes per lane, measured on MI355X, about as many
+//   as the ~20 VALU instructions of a sample: the kernel is balanced
+//   between the two);
 // * sample coordinates are evaluated directly, x_b(i) = a1*i + beta, instead
 //   of the reference's running sums, so the per-sample error does not grow
 //   along the ray;

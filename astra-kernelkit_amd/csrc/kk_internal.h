@@ -195,22 +195,13 @@ __device__ __forceinline__ float kk_clamp(float x, float lo, float hi) {
   return __builtin_amdgcn_fmed3f(x, lo, hi);
 }
 
-// Two consecutive floats, 4-byte aligned: one global_load_dwordx2.
-typedef float kk_float2u __attribute__((ext_vector_type(2), aligned(4)));
-
 // Loads at a wave-uniform base plus a 32-bit per-lane byte offset: the
 // compiler emits the SGPR-base ("saddr") form of global loads, so no 64-bit
 // address arithmetic is spent per lane.
-__device__ __forceinline__ kk_float2u kk_ld2(const char* base, uint32_t off) {
-  return *reinterpret_cast<const kk_float2u*>(base + off);
-}
 // Four consecutive floats, 8-byte aligned: one global_load_dwordx4.
 typedef float kk_float4u __attribute__((ext_vector_type(4), aligned(8)));
 __device__ __forceinline__ kk_float4u kk_ld4(const char* base, uint32_t off) {
   return *reinterpret_cast<const kk_float4u*>(base + off);
-}
-__device__ __forceinline__ float kk_ld1(const char* base, uint32_t off) {
-  return *reinterpret_cast<const float*>(base + off);
 }
 
 __device__ __forceinline__ int32_t kk_wave_min(int32_t v) {

@@ -217,7 +217,7 @@ class BatchSirt:
                 0.0 if max_constraint is None else float(max_constraint),
                 0 if min_constraint is None else 1,
                 0 if max_constraint is None else 1,
-                _native._vp(xtex.ptr), xtex.mask,
+                None, 0,   # frames of x: rebuilt after the update
                 float(vg.voxel_volume())))
             self.x.append(x)
             self.C.append(C)
@@ -243,13 +243,18 @@ class BatchSirt:
         self.reset()
 
     def iterate(self):
-        """One SIRT iteration of every problem: two launches."""
+        """One SIRT iteration of every problem: one persistent launch per
+        operator, then the refresh of each x's forward-projection frames
+        (coalesced texture kernels; the update epilogue's scattered frame
+        stores cost several times more)."""
         if self.cfg_fp is None:
             self.configure(tune=False)
         lib = _native.lib()
         st = _native.stream_ptr()
         _native.check(lib.kk_batch_fp(self.handle.ptr, self.cfg_fp.grid, st))
         _native.check(lib.kk_batch_bp(self.handle.ptr, self.cfg_bp.grid, st))
+        for t, x in zip(self._xtex, self.x):
+            t.update(x)
 
     def capture(self, iters: int):
         """Capture ``iters`` iterations in a HIP graph (torch.cuda.graph);
