@@ -8,14 +8,16 @@
 //   transformed, and its first `cols` entries kept.  The reference does this
 //   in float64 with cuFFT, one Python-level loop iteration per angle.
 //
-// MI355X design: one workgroup filters two detector rows at once.  They are
+// MI355X design: one workgroup filters two detector rows at once (radix-4
+// passes in registers, twiddles staged in LDS).  They are
 // packed as the real and imaginary parts of one complex signal; because H
 // is real and even, the real part of the result is the filtered first row
 // and the imaginary part the filtered second row.  The transform runs in
-// LDS: a radix-2 decimation-in-frequency pass (natural -> bit-reversed
-// order), the multiplication by H stored in bit-reversed order with 1/N
-// folded in, and a decimation-in-time inverse pass (bit-reversed ->
-// natural), so no explicit permutation is needed.  Twiddles and H come from
+// LDS: decimation-in-frequency passes (natural -> bit-reversed order;
+// radix 4, plus one radix-2 stage when log2 N is odd), the multiplication
+// by H stored in bit-reversed order with 1/N folded in, and
+// decimation-in-time inverse passes (bit-reversed -> natural), so no
+// explicit permutation is needed.  Twiddles and H come from
 // float64 host tables rounded once to float32.  The pre-weight can be fused
 // into the load, and the output can be written anywhere through strides
 // (e.g. straight into the backprojector's padded texture).
@@ -61,7 +63,7 @@ struct FilterArgs {
   int64_t os0, os1, os2;
   int64_t out_dual;  // if nonzero: also write out[o + out_dual]
   int64_t total_rows;
-  int32_t rows, cols, n, log2n;
+  int32_t rows, cols, n;
   const PreweightAngle* pw;  // optional fused pre-weight
   const float2* tw;          // exp(-2 pi i k / N), k < N/2
   const float* hb;           // H[bitrev(k)] / N
@@ -75,6 +77,9 @@ __device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
   return make_float2(fmaf(a.x, b.x, a.y * b.y), fmaf(a.y, b.x, -a.x * b.y));
 }
 
+// TW_LDS: the twiddles are copied into LDS after the signal (when both fit
+// in 64 KiB); otherwise they are read from global memory.
+template <bool TW_LDS>
 __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
   extern __shared__ float2 buf[];
   const int32_t N = A.n;
@@ -103,15 +108,46 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
   __syncthreads();
 
   const int32_t half = N >> 1;
-  // forward transform, decimation in frequency: natural -> bit-reversed
-  for (int32_t m = half, ts = 1; m >= 1; m >>= 1, ts <<= 1) {
+  const float2* tw = A.tw;
+  if constexpr (TW_LDS) {
+    float2* tl = buf + N;
+    for (int32_t k = threadIdx.x; k < half; k += 256) tl[k] = A.tw[k];
+    __syncthreads();
+    tw = tl;
+  }
+  const int32_t quarter = N >> 2;
+  // forward transform, decimation in frequency: natural -> bit-reversed.
+  // Pairs of radix-2 stages (spans m and m / 2) run as one radix-4 pass
+  // in registers; a last span-1 stage remains when log2(N) is odd.
+  int32_t m = half, ts = 1;
+  for (; m >= 2; m >>= 2, ts <<= 2) {
+    const int32_t q = m >> 1;
+    for (int32_t g = threadIdx.x; g < quarter; g += 256) {
+      const int32_t k = g & (q - 1);
+      const int32_t ia = ((g - k) << 2) + k;
+      const float2 xa = buf[ia], xb = buf[ia + q];
+      const float2 xc = buf[ia + m], xd = buf[ia + m + q];
+      // span m: (a, c) with twiddle k, (b, d) with twiddle k + q
+      const float2 a1 = make_float2(xa.x + xc.x, xa.y + xc.y);
+      const float2 c1 =
+          cmul(make_float2(xa.x - xc.x, xa.y - xc.y), tw[k * ts]);
+      const float2 b1 = make_float2(xb.x + xd.x, xb.y + xd.y);
+      const float2 d1 =
+          cmul(make_float2(xb.x - xd.x, xb.y - xd.y), tw[(k + q) * ts]);
+      // span q: (a, b) and (c, d), both with twiddle k of the next stage
+      const float2 w2 = tw[k * (ts << 1)];
+      buf[ia] = make_float2(a1.x + b1.x, a1.y + b1.y);
+      buf[ia + q] = cmul(make_float2(a1.x - b1.x, a1.y - b1.y), w2);
+      buf[ia + m] = make_float2(c1.x + d1.x, c1.y + d1.y);
+      buf[ia + m + q] = cmul(make_float2(c1.x - d1.x, c1.y - d1.y), w2);
+    }
+    __syncthreads();
+  }
+  if (m == 1) {  // span 1: twiddle 1
     for (int32_t j = threadIdx.x; j < half; j += 256) {
-      const int32_t k = j & (m - 1);
-      const int32_t i0 = ((j - k) << 1) + k;
-      const int32_t i1 = i0 + m;
-      const float2 x0 = buf[i0], x1 = buf[i1];
-      buf[i0] = make_float2(x0.x + x1.x, x0.y + x1.y);
-      buf[i1] = cmul(make_float2(x0.x - x1.x, x0.y - x1.y), A.tw[k * ts]);
+      const float2 x0 = buf[2 * j], x1 = buf[2 * j + 1];
+      buf[2 * j] = make_float2(x0.x + x1.x, x0.y + x1.y);
+      buf[2 * j + 1] = make_float2(x0.x - x1.x, x0.y - x1.y);
     }
     __syncthreads();
   }
@@ -122,16 +158,41 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
     buf[c] = make_float2(x.x * h, x.y * h);
   }
   __syncthreads();
-  // inverse transform, decimation in time: bit-reversed -> natural
-  for (int32_t m = 1, ts = half; m < N; m <<= 1, ts >>= 1) {
+  // inverse transform, decimation in time: bit-reversed -> natural; a
+  // first span-1 stage when log2(N) is odd, then radix-4 passes (spans
+  // m and 2 m)
+  m = 1;
+  ts = half;
+  if (__builtin_ctz(static_cast<uint32_t>(N)) & 1) {  // log2(N) odd
     for (int32_t j = threadIdx.x; j < half; j += 256) {
-      const int32_t k = j & (m - 1);
-      const int32_t i0 = ((j - k) << 1) + k;
-      const int32_t i1 = i0 + m;
-      const float2 x0 = buf[i0];
-      const float2 t = cmulc(buf[i1], A.tw[k * ts]);
-      buf[i0] = make_float2(x0.x + t.x, x0.y + t.y);
-      buf[i1] = make_float2(x0.x - t.x, x0.y - t.y);
+      const float2 x0 = buf[2 * j], x1 = buf[2 * j + 1];
+      buf[2 * j] = make_float2(x0.x + x1.x, x0.y + x1.y);
+      buf[2 * j + 1] = make_float2(x0.x - x1.x, x0.y - x1.y);
+    }
+    __syncthreads();
+    m = 2;
+    ts = half >> 1;
+  }
+  for (; m < N; m <<= 2, ts >>= 2) {
+    for (int32_t g = threadIdx.x; g < quarter; g += 256) {
+      const int32_t k = g & (m - 1);
+      const int32_t ia = ((g - k) << 2) + k;
+      const float2 xa = buf[ia], xb = buf[ia + m];
+      const float2 xc = buf[ia + 2 * m], xd = buf[ia + 3 * m];
+      // span m: (a, b) and (c, d), both with twiddle k
+      const float2 w1 = tw[k * ts];
+      const float2 tb = cmulc(xb, w1), td = cmulc(xd, w1);
+      const float2 a1 = make_float2(xa.x + tb.x, xa.y + tb.y);
+      const float2 b1 = make_float2(xa.x - tb.x, xa.y - tb.y);
+      const float2 c1 = make_float2(xc.x + td.x, xc.y + td.y);
+      const float2 d1 = make_float2(xc.x - td.x, xc.y - td.y);
+      // span 2 m: (a, c) with twiddle k, (b, d) with twiddle k + m
+      const float2 tc = cmulc(c1, tw[k * (ts >> 1)]);
+      const float2 tdd = cmulc(d1, tw[(k + m) * (ts >> 1)]);
+      buf[ia] = make_float2(a1.x + tc.x, a1.y + tc.y);
+      buf[ia + 2 * m] = make_float2(a1.x - tc.x, a1.y - tc.y);
+      buf[ia + m] = make_float2(b1.x + tdd.x, b1.y + tdd.y);
+      buf[ia + 3 * m] = make_float2(b1.x - tdd.x, b1.y - tdd.y);
     }
     __syncthreads();
   }
@@ -320,9 +381,13 @@ static int filter_rows(const kk_filter* f, const float* in,
   const int64_t pairs = (A.total_rows + 1) / 2;
   if (pairs >= (int64_t(1) << 31))
     return kk::fail(KK_ERR_UNSUPPORTED, "kk_filter_rows: too many rows");
-  const size_t lds = sizeof(float2) * size_t(f->n);
-  hipLaunchKernelGGL(filter_rows_kernel, dim3(unsigned(pairs)), dim3(256),
-                     lds, stream, A);
+  const size_t lds_tw = sizeof(float2) * (size_t(f->n) + size_t(f->n) / 2);
+  if (lds_tw <= 65536)
+    hipLaunchKernelGGL(filter_rows_kernel<true>, dim3(unsigned(pairs)),
+                       dim3(256), lds_tw, stream, A);
+  else
+    hipLaunchKernelGGL(filter_rows_kernel<false>, dim3(unsigned(pairs)),
+                       dim3(256), sizeof(float2) * size_t(f->n), stream, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kk::hip_fail(e, "filter_rows_kernel");
   return KK_OK;

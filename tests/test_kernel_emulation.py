@@ -493,3 +493,66 @@ def test_fp_unclamped_interior_is_inside(name, angles):
             assert np.all((xb >= 0.5) & (xb <= nb + 0.5)), name
             assert np.all((xc >= 0.5) & (xc <= nc + 0.5)), name
     assert checked > 0
+
+
+def _emulate_filter_fft(x, he_bitrev):
+    """filter_rows_kernel's transform in numpy (float64, same index
+    arithmetic): radix-4 decimation-in-frequency passes (plus a span-1
+    stage for odd log2 N), multiplication by the bit-reversed response,
+    radix-4 decimation-in-time passes (plus a first span-1 stage)."""
+    buf = np.array(x, dtype=np.complex128)
+    N = buf.size
+    half, quarter = N // 2, N // 4
+    log2n = N.bit_length() - 1
+    tw = np.exp(-2j * np.pi * np.arange(half) / N)
+    g = np.arange(quarter)
+    m, ts = half, 1
+    while m >= 2:
+        q = m // 2
+        k = g & (q - 1)
+        ia = ((g - k) << 2) + k
+        xa, xb, xc, xd = buf[ia], buf[ia + q], buf[ia + m], buf[ia + m + q]
+        a1, c1 = xa + xc, (xa - xc) * tw[k * ts]
+        b1, d1 = xb + xd, (xb - xd) * tw[(k + q) * ts]
+        w2 = tw[k * (ts << 1)]
+        buf[ia], buf[ia + q] = a1 + b1, (a1 - b1) * w2
+        buf[ia + m], buf[ia + m + q] = c1 + d1, (c1 - d1) * w2
+        m, ts = m >> 2, ts << 2
+    if m == 1:
+        x0, x1 = buf[0::2].copy(), buf[1::2].copy()
+        buf[0::2], buf[1::2] = x0 + x1, x0 - x1
+    buf *= he_bitrev
+    m, ts = 1, half
+    if log2n & 1:
+        x0, x1 = buf[0::2].copy(), buf[1::2].copy()
+        buf[0::2], buf[1::2] = x0 + x1, x0 - x1
+        m, ts = 2, half >> 1
+    while m < N:
+        k = g & (m - 1)
+        ia = ((g - k) << 2) + k
+        xa, xb = buf[ia], buf[ia + m]
+        xc, xd = buf[ia + 2 * m], buf[ia + 3 * m]
+        w1 = np.conj(tw[k * ts])
+        tb, td = xb * w1, xd * w1
+        a1, b1, c1, d1 = xa + tb, xa - tb, xc + td, xc - td
+        tc = c1 * np.conj(tw[k * (ts >> 1)])
+        tdd = d1 * np.conj(tw[(k + m) * (ts >> 1)])
+        buf[ia], buf[ia + 2 * m] = a1 + tc, a1 - tc
+        buf[ia + m], buf[ia + 3 * m] = b1 + tdd, b1 - tdd
+        m, ts = m << 2, ts >> 2
+    return buf
+
+
+@pytest.mark.parametrize("N", [4, 8, 64, 512, 1024, 2048, 4096])
+def test_filter_radix4_index_arithmetic(N):
+    """The row filter's radix-4 passes equal circular convolution with the
+    (even) response: IDFT(DFT(x) He) for a complex row (two real rows)."""
+    rng = np.random.default_rng(N)
+    x = rng.normal(size=N) + 1j * rng.normal(size=N)
+    H = rng.uniform(size=N)
+    he = 0.5 * (H + H[(-np.arange(N)) % N])
+    bits = N.bit_length() - 1
+    rev = np.array([int(format(i, f"0{bits}b")[::-1], 2) for i in range(N)])
+    got = _emulate_filter_fft(x, he[rev] / N)
+    ref = np.fft.ifft(np.fft.fft(x) * he)
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(x).max())
