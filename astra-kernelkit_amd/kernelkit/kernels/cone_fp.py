@@ -337,9 +337,10 @@ class RayDrivenConeFP(BaseKernel):
         backprojector, ``torch_support.py:29-33``).  ``projections`` is a
         device tensor in ``projection_axes`` order, ``volume`` one in
         ``volume_axes`` order.  Every (ray, slice) sample of :meth:`__call__`
-        adds its weighted value to its bilinear taps, with the forward
-        kernel's float32 arithmetic, so ``<A x, y> = <x, A^T y>`` up to
-        rounding.
+        adds its weighted value to its bilinear taps, with weights and
+        coordinates computed by the forward kernel's own functions (float64
+        coordinates, bit-identical weights), so ``<A x, y> = <x, A^T y>`` up
+        to rounding.
         """
         if not self.is_compiled:
             raise KernelNotCompiledException(

@@ -5,7 +5,9 @@
  * device-side call is asynchronous on the given HIP stream (a hipStream_t
  * passed as void*; NULL = the null stream), allocates nothing and never
  * synchronises, so it can be captured in a HIP graph.  Only the
- * kk_set_params_* calls allocate (a per-plan device table) and synchronise.
+ * kk_set_params_*, kk_filter_set_* and kk_batch_configure calls allocate (a
+ * small device table) and synchronise; scratch that a launch needs is passed
+ * in by the caller (kk_cone_fp_transpose's work).
  *
  * Errors: every int-returning function returns KK_OK (0) or a negative
  * KK_ERR_* code, and records a message for kk_last_error() (thread-local).
@@ -126,8 +128,9 @@ int kk_cone_bp(const kk_plan* plan, const float* proj_tex, int32_t n_angles,
 
 /* Exact transpose of kk_cone_fp (matched adjoint, for gradients): every
  * (ray, slice) sample of the forward projection adds proj * scale *
- * bilinear weight to its in-volume taps, with the forward kernel's float32
- * arithmetic, so <A x, y> = <x, A^T y> up to rounding.  No counterpart in
+ * bilinear weight to its in-volume taps, with the forward kernel's own
+ * coordinate and weight arithmetic (bit-identical weights), so
+ * <A x, y> = <x, A^T y> up to rounding.  No counterpart in
  * the reference, whose autograd backward uses the voxel-driven
  * backprojector (kernelkit/torch_support.py:29-33); it replaces that call
  * where an exact gradient is wanted.  proj: (angle, row, col) strides; vol:

@@ -107,6 +107,39 @@ def test_projection_axes(axes, device):
     assert_parity(got_bp, ref_bp, label=f"bp/projection_axes={axes}")
 
 
+@pytest.fixture(scope="module")
+def axis_matrix_case():
+    """A perturbed orbit (per-angle detector tilt and shift) with its oracle
+    FP, BP and exact transpose in world order, computed once."""
+    pgs, vg, vol = _small_case(n_angles=8, shape=(20, 24, 28), rows=32,
+                               cols=36, tilt_deg=2.0, drift_px=3.0)
+    fp = oracle.cone_fp(vol, pgs, vg)
+    bp = oracle.cone_bp(fp, pgs, vg)
+    y = np.random.default_rng(9).uniform(size=fp.shape)
+    fpt = oracle.cone_fp_transpose(y, pgs, vg)
+    return pgs, vg, vol, fp, bp, y, fpt
+
+
+@pytest.mark.parametrize("vaxes", list(itertools.permutations((0, 1, 2))))
+@pytest.mark.parametrize("paxes", list(itertools.permutations((0, 1, 2))))
+def test_axis_order_matrix(axis_matrix_case, vaxes, paxes, device):
+    """Every (volume_axes, projection_axes) pair, ASTRA's (2, 1, 0) /
+    (1, 0, 2) among them (the reference specialises its kernels per pair,
+    kernels/cone_fp.py:43-52, cuda/cone_fp.cu:94-135, cuda/cone_bp.cu:89-91;
+    here they are strides): FP, BP and the exact transpose."""
+    pgs, vg, vol, fp, bp, y, fpt = axis_matrix_case
+    A = kk.XrayTransform(pgs, vg, volume_axes=vaxes, projection_axes=paxes)
+    got = _host(A(_dev(np.transpose(vol, vaxes))))
+    assert_parity(np.transpose(got, np.argsort(paxes)), fp,
+                  label=f"fp/{vaxes}/{paxes}")
+    got = _host(A.T(_dev(np.transpose(fp, paxes))))
+    assert_parity(np.transpose(got, np.argsort(vaxes)), bp,
+                  label=f"bp/{vaxes}/{paxes}")
+    got = _host(A.matched_T(_dev(np.transpose(y, paxes))))
+    assert_parity(np.transpose(got, np.argsort(vaxes)), fpt,
+                  label=f"fp_t/{vaxes}/{paxes}")
+
+
 def test_shifted_rotated_volume(device):
     pgs, _, _ = _small_case(n_angles=12)
     shape = (30, 34, 26)
