@@ -340,9 +340,28 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       }
       for (int32_t col = c_lo; col <= c_hi; ++col) {
         const float* ycol = img + col * A.rows;
+        // the column terms of kk_fp_ray; its row terms are added per row in
+        // the same nested order, so every value is bit-identical to it
+        const double fc = static_cast<double>(col);
+        const double Da0 = fma(fc, P.ua, P.qa);
+        const double Db0 = fma(fc, P.ub, P.qb);
+        const double Dc0 = fma(fc, P.uc, P.qc);
+        // b-tap sums over a run of samples with the same tap index, added
+        // to the lane's LDS words when the index changes (along a column
+        // it rarely does)
+        int32_t tcur = -1;
+        float s0[kJC], s1[kJC];
+#pragma unroll
+        for (int cc = 0; cc < kJC; ++cc) s0[cc] = s1[cc] = 0.f;
         for (int32_t r = r_lo; r <= r_hi; ++r) {
-          double a1, b1, a2, b2;
-          if (!kk_fp_ray(P, r, col, a1, b1, a2, b2)) continue;
+          const double fr = static_cast<double>(r);
+          const double inv = 1.0 / fma(fr, P.va, Da0);
+          const double a1 = fma(fr, P.vb, Db0) * inv;
+          const double a2 = fma(fr, P.vc, Dc0) * inv;
+          const double b1 = fma(a1, P.sig, P.taub);
+          const double b2 = fma(a2, P.sig, P.tauc);
+          // rays kk_fp_ray rejects (non-finite terms) have yw = 0, and
+          // their clamped coordinates keep every index in range
           uint32_t ic;
           float fcw;
           kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
@@ -359,13 +378,27 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
           kk_fp_coord(a1, b1, fi, nb1, ib, fb);
           const int32_t t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
           if (t0 < -1 || t0 >= kJB) continue;
+          if (t0 != tcur) {
+#pragma unroll
+            for (int cc = 0; cc < kJC; ++cc) {
+              g[tcur + 1][cc][lane] += s0[cc];
+              g[tcur + 2][cc][lane] += s1[cc];
+              s0[cc] = s1[cc] = 0.f;
+            }
+            tcur = t0;
+          }
           const float yv = ycol[r];
 #pragma unroll
           for (int cc = 0; cc < kJC; ++cc) {
             const float val = yv * wk[cc];
-            g[t0 + 1][cc][lane] = fmaf(1.f - fb, val, g[t0 + 1][cc][lane]);
-            g[t0 + 2][cc][lane] = fmaf(fb, val, g[t0 + 2][cc][lane]);
+            s0[cc] = fmaf(1.f - fb, val, s0[cc]);
+            s1[cc] = fmaf(fb, val, s1[cc]);
           }
+        }
+#pragma unroll
+        for (int cc = 0; cc < kJC; ++cc) {
+          g[tcur + 1][cc][lane] += s0[cc];
+          g[tcur + 2][cc][lane] += s1[cc];
         }
       }
     }
