@@ -155,7 +155,21 @@ def lib():
         fn.restype = res
         fn.argtypes = args
     _lib = handle
+    st = path.stat()
+    global _build_id
+    _build_id = (f"{handle.kk_version().decode()} "
+                 f"{st.st_size}:{st.st_mtime_ns}")
     return _lib
+
+
+_build_id = None
+
+
+def build_id() -> str:
+    """Version string plus size and mtime of the loaded library file (changes
+    with every rebuild)."""
+    lib()
+    return _build_id
 
 
 def last_error() -> str:

@@ -21,8 +21,15 @@ from kernelkit import _native
 
 FP_ROWS = (1, 2, 4)
 BP_ZRUN = (4, 8, 16)
-N_CU = 256
-GRIDS = (0, 4 * N_CU, 8 * N_CU)
+GRIDS_PER_CU = (0, 4, 8)   # persistent workgroups per CU (0: one per tile)
+
+
+def _grids():
+    """Candidate persistent grid sizes for the current device."""
+    import torch
+    n_cu = torch.cuda.get_device_properties(
+        torch.cuda.current_device()).multi_processor_count
+    return tuple(g * n_cu for g in GRIDS_PER_CU)
 
 
 @dataclass(frozen=True)
@@ -44,19 +51,25 @@ class BatchTuner:
                 with open(self.cache_file) as f:
                     for k, v in json.load(f).items():
                         self._cache[k] = tuple(LaunchConfig(**c) for c in v)
-            except (OSError, ValueError, TypeError):
-                self._cache = {}
+            except (OSError, ValueError, TypeError, AttributeError):
+                self._cache = {}     # unreadable or foreign file: start empty
 
     def _store(self, key, cfgs):
         self._cache[key] = cfgs
         if self.cache_file:
             data = {k: [c.__dict__ for c in v] for k, v in self._cache.items()}
-            with open(self.cache_file, "w") as f:
+            tmp = f"{self.cache_file}.{os.getpid()}.tmp"
+            with open(tmp, "w") as f:
                 json.dump(data, f)
+            os.replace(tmp, self.cache_file)   # readers never see a partial file
 
     @staticmethod
     def _key(signature) -> str:
-        return repr(signature)
+        """Cache key: the batch signature, the library build and the device
+        (timings do not carry over between builds or devices)."""
+        import torch
+        dev = torch.cuda.get_device_name(torch.cuda.current_device())
+        return repr((_native.build_id(), dev, signature))
 
     def _time(self, launch) -> float:
         import torch
@@ -78,7 +91,7 @@ class BatchTuner:
         best = None
         tiles = FP_ROWS if which == "fp" else BP_ZRUN
         fn = lib.kk_batch_fp if which == "fp" else lib.kk_batch_bp
-        for tile, grid in product(tiles, GRIDS):
+        for tile, grid in product(tiles, _grids()):
             rpt, zrun = (tile, other_tile) if which == "fp" else \
                 (other_tile, tile)
             _native.check(lib.kk_batch_configure(handle.ptr, rpt, zrun))

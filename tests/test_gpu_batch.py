@@ -63,6 +63,29 @@ def test_batch_fp_bp_match_single(tune, device):
                                    atol=1e-6 * float(ref.abs().max()))
 
 
+def test_tuner_cache_file_round_trip(tmp_path, device):
+    """A tuned choice is written atomically to the cache file and reused by
+    a second tuner without timing again; keys carry the library build."""
+    import json
+    from kernelkit import _native
+    from kernelkit.autotune import BatchTuner
+    from kernelkit.batch import ConeBatch
+    path = tmp_path / "tune.json"
+    P = _problems(3, seed=2)
+    B = ConeBatch(P, tuner=BatchTuner(reps=1, cache_file=str(path)))
+    vols = [_dev(v) for v in _volumes(P)]
+    ys = B.fp(vols, tune=True)
+    data = json.loads(path.read_text())
+    assert len(data) == 1 and _native.build_id() in next(iter(data))
+    assert not list(tmp_path.glob("*.tmp"))
+    t2 = BatchTuner(reps=1, cache_file=str(path))
+    t2._tune = None          # a cache hit must not time anything
+    B2 = ConeBatch(P, tuner=t2)
+    ys2 = B2.fp(vols, tune=True)
+    for a, b in zip(ys, ys2):
+        assert bool((a == b).all())
+
+
 def test_batch_matches_oracle(device):
     from kernelkit.batch import ConeBatch
     # a few angles each keep the oracle fast
