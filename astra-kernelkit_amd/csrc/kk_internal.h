@@ -245,25 +245,40 @@ __device__ __forceinline__ float kk_fp_weight(const kk::FpAngle& P,
   return sqrtf(fmaf(f1 * f1, P.sc1, fmaf(f2 * f2, P.sc2, 1.f))) * P.oscale;
 }
 
+// (double)i for a wave-uniform 0 <= i < 2^31, assembled from the bits with
+// scalar integer instructions (exponent from the leading-one position), so
+// the vector unit does not convert the loop counter every slice.
+__device__ __forceinline__ double kk_uniform_i2d(int32_t i) {
+  const uint32_t u = static_cast<uint32_t>(i);
+  const uint32_t e = 31u - static_cast<uint32_t>(__builtin_clz(u | 1u));
+  const uint64_t bits = (static_cast<uint64_t>(u) << (52u - e)) +
+                        (static_cast<uint64_t>(1022u + e) << 52);
+  return u == 0u ? 0.0 : __builtin_bit_cast(double, bits);
+}
+
+// Split a coordinate x >= 0 into floor(x) and x - floor(x): for x >= 0 the
+// truncating conversion is the floor, and v_fract_f64 returns x - floor(x)
+// exactly (the difference is representable), one instruction fewer than
+// floor + convert + subtract.
+__device__ __forceinline__ void kk_split_nonneg(double x, uint32_t& idx,
+                                                float& frac) {
+  idx = static_cast<uint32_t>(x);
+  frac = static_cast<float>(__builtin_amdgcn_fract(x));
+}
+
 // Sample coordinate a * i + b clamped into [0, n1] (the zero halo), split
 // into the padded element index and the interpolation weight.  float64:
 // coordinates reach ~1e3, where float32 would be off by ~1e-4.
 __device__ __forceinline__ void kk_fp_coord(double a, double b, double fi,
                                             double n1, uint32_t& idx,
                                             float& frac) {
-  const double x = fmin(fmax(fma(a, fi, b), 0.0), n1);
-  const double fl = floor(x);
-  idx = static_cast<uint32_t>(fl);
-  frac = static_cast<float>(x - fl);
+  kk_split_nonneg(fmin(fmax(fma(a, fi, b), 0.0), n1), idx, frac);
 }
 
 // kk_fp_coord for a coordinate known to lie in [0, n1]: the same index and
 // weight without the clamp.
 __device__ __forceinline__ void kk_fp_coord_in(double a, double b, double fi,
                                                uint32_t& idx, float& frac) {
-  const double x = fma(a, fi, b);
-  const double fl = floor(x);
-  idx = static_cast<uint32_t>(fl);
-  frac = static_cast<float>(x - fl);
+  kk_split_nonneg(fma(a, fi, b), idx, frac);
 }
 
