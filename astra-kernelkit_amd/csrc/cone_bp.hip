@@ -73,15 +73,16 @@ __device__ __forceinline__ double refine_rcp(double den, float rf) {
 // Coordinate base + off (base an integer, off a small float) clamped into
 // [0, hi] (the zero halo), split into element index and weight: the same
 // result as clamping the float64 coordinate, then floor and fraction.
+// The clamp is applied to off, into [lo, up] = [-base, hi - base] (integers,
+// exact in float32 for |base|, hi < 2^24; the caller passes them
+// converted), so at either bound the weight is 0 and the index 0 or hi.
 __device__ __forceinline__ void split_clamped(float off, int32_t base,
-                                              int32_t hi, int32_t& idx,
-                                              float& frac) {
-  const float o = kk_clamp(off, -1e7f, 1e7f);
+                                              float lo, float up,
+                                              int32_t& idx, float& frac) {
+  const float o = __builtin_amdgcn_fmed3f(off, lo, up);
   const float fl = floorf(o);
-  const int32_t i = base + static_cast<int32_t>(fl);
-  const bool out = i < 0 || i >= hi;
-  frac = out ? 0.f : o - fl;
-  idx = min(max(i, 0), hi);
+  idx = base + static_cast<int32_t>(fl);
+  frac = o - fl;
 }
 
 // split_clamped for a coordinate known to lie in [0, hi): no clamps, the
@@ -174,12 +175,16 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
                    (static_cast<uint32_t>(iu) << 3);
         }
       } else {
+        const float ulo = static_cast<float>(-iub);
+        const float uup = static_cast<float>(cmax - iub);
+        const float vlo = static_cast<float>(-ivb);
+        const float vup = static_cast<float>(rmax - ivb);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const float fk = static_cast<float>(k);
           int32_t iu, iv;
-          split_clamped(fmaf(fk, gu, fu0), iub, cmax, iu, fu[k]);
-          split_clamped(fmaf(fk, gv, fv0), ivb, rmax, iv, fv[k]);
+          split_clamped(fmaf(fk, gu, fu0), iub, ulo, uup, iu, fu[k]);
+          split_clamped(fmaf(fk, gv, fv0), ivb, vlo, vup, iv, fv[k]);
           off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
                    (static_cast<uint32_t>(iu) << 3);
         }
@@ -201,20 +206,22 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
       // (den = 0) has no projection: skip it
       if (!(isfinite(u) && isfinite(v0) && isfinite(gv))) continue;
       const float w = rf * rf;
-      const double uc = fmin(fmax(u, 0.0), static_cast<double>(cmax));
-      const double ufl = floor(uc);
-      const float fu = static_cast<float>(uc - ufl);
-      const uint32_t offu = static_cast<uint32_t>(ufl) << 3;
+      uint32_t iu;
+      float fu;
+      kk_split_nonneg(fmin(fmax(u, 0.0), static_cast<double>(cmax)), iu, fu);
+      const uint32_t offu = iu << 3;
       const double vc0 = fmin(fmax(v0, -1e6), 1e6);
       const double vfl = floor(vc0);
       const int32_t ivb = static_cast<int32_t>(vfl);
       const float fv0 = static_cast<float>(vc0 - vfl);
+      const float vlo = static_cast<float>(-ivb);
+      const float vup = static_cast<float>(rmax - ivb);
       uint32_t off[K];
       float fv[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         int32_t iv;
-        split_clamped(fmaf(static_cast<float>(k), gv, fv0), ivb, rmax, iv,
+        split_clamped(fmaf(static_cast<float>(k), gv, fv0), ivb, vlo, vup, iv,
                       fv[k]);
         off[k] = __umul24(static_cast<uint32_t>(iv), trb) + offu;
       }
@@ -276,6 +283,12 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
           w[k] = r * r;
         }
       } else {
+        // a voxel without a finite projection reads index 0 with weight 0
+        // (its offset is put at the lower bound)
+        const float ulo = static_cast<float>(-iub);
+        const float uup = static_cast<float>(cmax - iub);
+        const float vlo = static_cast<float>(-ivb);
+        const float vup = static_cast<float>(rmax - ivb);
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const float fk = static_cast<float>(k);
@@ -284,8 +297,8 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
           const float dv = fmaf(fk, e1v, e0v) * r;
           const bool ok = isfinite(du) && isfinite(dv);
           int32_t iu, iv;
-          split_clamped(ok ? du : 0.f, ok ? iub : 0, cmax, iu, fu[k]);
-          split_clamped(ok ? dv : 0.f, ok ? ivb : 0, rmax, iv, fv[k]);
+          split_clamped(ok ? du : ulo, iub, ulo, uup, iu, fu[k]);
+          split_clamped(ok ? dv : vlo, ivb, vlo, vup, iv, fv[k]);
           off[k] = __umul24(static_cast<uint32_t>(iv), trb) +
                    (static_cast<uint32_t>(iu) << 3);
           w[k] = ok ? r * r : 0.f;

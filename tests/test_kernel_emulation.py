@@ -90,12 +90,55 @@ def _coord(a, b, i, n1):
 
 
 def _split_clamped(off, base, hi):
-    """split_clamped of cone_bp.hip: base + off clamped into [0, hi]."""
+    """split_clamped of cone_bp.hip: off clamped into [-base, hi - base]
+    (float32 med3), so base + off lies in [0, hi]; then floor and
+    fraction."""
+    base = np.asarray(base, np.int64)
+    o = np.clip(off.astype(f32), (-base).astype(f32),
+                (hi - base).astype(f32)).astype(f32)
+    fl = np.floor(o)
+    return base + fl.astype(np.int64), (o - fl).astype(f32)
+
+
+def _split_clamped_flagged(off, base, hi):
+    """The earlier form of split_clamped: clamp the float32 offset to
+    +-1e7, split, then flag indices outside [0, hi) (weight 0, index
+    clamped)."""
     o = np.clip(off.astype(f32), f32(-1e7), f32(1e7))
     fl = np.floor(o)
     i = base + fl.astype(np.int64)
     out = (i < 0) | (i >= hi)
     return np.clip(i, 0, hi), np.where(out, f32(0), (o - fl).astype(f32))
+
+
+def test_split_clamped_equals_clamp_then_split():
+    """The offset clamp into [-base, hi - base] gives the index and weight
+    of clamping the exact coordinate base + off into [0, hi] (float64,
+    exact for float32 offsets and integer bases) and of the flagged form,
+    bit for bit, at the bounds, near integers and for huge offsets."""
+    rng = np.random.default_rng(11)
+    n = 200000
+    hi = rng.integers(40, 1600, n)
+    base = rng.integers(-3000, 3000, n)
+    off = np.concatenate([
+        rng.uniform(-4000, 4000, n // 4),
+        rng.uniform(-3, 3, n // 4),
+        np.round(rng.uniform(-3000, 3000, n // 4)),        # integers
+        rng.choice([1e9, -1e9, 1e7, -1e7, 2.0 ** -30, -2.0 ** -30,
+                    -1e-9, 1 - 2.0 ** -24], n // 4),
+    ]).astype(f32)
+    # offsets that land exactly on the bounds 0 and hi
+    off[:1000] = (-base[:1000]).astype(f32)
+    off[1000:2000] = (hi[1000:2000] - base[1000:2000]).astype(f32)
+    idx, frac = _split_clamped(off, base, hi)
+    x = np.clip(base + off.astype(np.float64), 0.0, hi.astype(np.float64))
+    fl = np.floor(x)
+    np.testing.assert_array_equal(idx, fl.astype(np.int64))
+    np.testing.assert_array_equal(frac, (x - fl).astype(f32))
+    idx2, frac2 = _split_clamped_flagged(off, base, hi)
+    np.testing.assert_array_equal(idx, idx2)
+    np.testing.assert_array_equal(frac, frac2)
+    assert idx.min() >= 0 and (idx <= hi).all()
 
 
 def emulate_bp(proj, table, shape, scale):
@@ -176,12 +219,11 @@ def emulate_bp(proj, table, shape, scale):
                 du = ((kz * e1u + e0u) * rk).astype(f32)
                 dv = ((kz * e1v + e0v) * rk).astype(f32)
                 ok = np.isfinite(du) & np.isfinite(dv)
-                iu, fu = _split_clamped(np.where(ok, du, f32(0)),
-                                        np.where(ok, ub, 0).astype(np.int64),
-                                        cols + 1)
-                iv, fv = _split_clamped(np.where(ok, dv, f32(0)),
-                                        np.where(ok, vb, 0).astype(np.int64),
-                                        rows + 1)
+                # no finite projection: offset at the lower bound
+                iu, fu = _split_clamped(np.where(ok, du, -ub).astype(f32),
+                                        ub.astype(np.int64), cols + 1)
+                iv, fv = _split_clamped(np.where(ok, dv, -vb).astype(f32),
+                                        vb.astype(np.int64), rows + 1)
                 w = np.where(ok, rk * rk, f32(0))
         img = tex[k + lo]
         h0 = fv * (img[iv + 1, iu] - img[iv, iu]) + img[iv, iu]
