@@ -466,7 +466,7 @@ int make_bp_args(const kk::BpSpec& S, int32_t zrun, BpArgs& A) {
   if (A.total_blocks >= (int64_t(1) << 31))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: grid too large");
   const int32_t plane = A.tiles_x * A.tiles_y;
-  A.xcd_group = (plane % kNumXcd == 0) ? plane / kNumXcd : 1;
+  A.xcd_group = 1;
   return KK_OK;
 }
 

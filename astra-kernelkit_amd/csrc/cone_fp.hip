@@ -38,8 +38,12 @@
 //   b-coordinate, its tap row and weight are shared by the thread's R rows
 //   ("column-coherent" angles, flagged per angle on the host);
 // * per-angle parameters are wave-uniform (scalar loads); workgroups are
-//   ordered row band -> angle -> column tile, and grouped so that each XCD
-//   keeps the same column range across consecutive angles (L2 reuse).
+//   ordered row band -> column tile -> angle, and runs of kXcdRun
+//   consecutive ids (consecutive angles of one column tile) go to one XCD,
+//   which holds about that many such workgroups at a time: neighbouring
+//   angles read nearly the same texture lines, so its L2 serves them
+//   (measured on MI355X at 512^3 / 720 angles: 90 ms, against 110 ms with
+//   column tiles fastest and workgroups dealt round-robin to the XCDs).
 #include <new>
 #include <vector>
 
@@ -51,10 +55,13 @@ constexpr int kWave = 64;
 // Detector columns per workgroup, one per wavefront.  Single problems: 16
 // (1024 threads); the wavefronts of a workgroup share one CU's L1 and move
 // through the slices nearly in step, so neighbouring columns' taps hit
-// there (measured on MI355X at 512^3 / 720 angles: 4.5% faster than 4 or
-// 8 columns, the same at 256^3).  Batches of small detectors: 4.
+// there (measured on MI355X at 512^3 / 720 angles: 3-8% faster than 8 or
+// 4 columns).  Batches of small detectors: 4.
 constexpr int kColsSingle = 16;
 constexpr int kColsBatch = 4;
+// Consecutive logical workgroup ids given to one XCD: 32 CUs per XCD x 2
+// resident workgroups of 1024 threads.
+constexpr int32_t kXcdRun = 64;
 
 struct FpArgs {
   const char* tex;  // frame textures (byte base)
@@ -239,10 +246,10 @@ template <int R, bool ALL_COLC, bool PIPE, int COLS>
 __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
                                         const FpArgs& A, int64_t bid) {
   constexpr int NB = ALL_COLC ? 1 : R;
-  const int32_t ct = static_cast<int32_t>(bid % A.col_tiles);
-  const int64_t t = bid / A.col_tiles;
-  const int32_t ang = static_cast<int32_t>(t % A.n_angles);
-  const int32_t rt = static_cast<int32_t>(t / A.n_angles);
+  const int32_t ang = static_cast<int32_t>(bid % A.n_angles);
+  const int64_t t = bid / A.n_angles;
+  const int32_t ct = static_cast<int32_t>(t % A.col_tiles);
+  const int32_t rt = static_cast<int32_t>(t / A.col_tiles);
   const int32_t lane = int32_t(threadIdx.x) & (kWave - 1);
   const int32_t col = ct * COLS + int32_t(threadIdx.x) / kWave;
   const int32_t row0 = rt * (kWave * R) + lane;
@@ -428,8 +435,7 @@ int make_fp_args(const kk::FpSpec& S, int32_t rpt, int32_t cols_per_block,
   A.total_blocks = int64_t(A.row_tiles) * A.n_angles * A.col_tiles;
   if (A.total_blocks >= (int64_t(1) << 31))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_fp: grid too large");
-  A.xcd_group = (A.col_tiles % kk::kNumXcd == 0) ? A.col_tiles / kk::kNumXcd
-                                                  : 1;
+  A.xcd_group = kXcdRun;
   return KK_OK;
 }
 
