@@ -104,8 +104,10 @@ __device__ __forceinline__ bool run_inside(int32_t base, float lo,
 }
 
 // One workgroup's tile (16 x 16 columns of K voxels) of logical block id
-// `bid`.
-template <int K>
+// `bid`.  TABLE_IN_MEMORY: the angle table pointer was read from a
+// descriptor (batch kernel), so each record is copied through
+// constant-memory (scalar) loads; a kernel-argument table is read in place.
+template <int K, bool TABLE_IN_MEMORY>
 __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
                                         const BpArgs& A, int64_t bid) {
   const int32_t tx = static_cast<int32_t>(bid % A.tiles_x);
@@ -134,7 +136,9 @@ __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
 
   const char* img = A.tex;
   for (int32_t a = 0; a < A.n_angles; ++a, img += A.tab_b) {
-    const kk::BpAngle& P = tab[a];
+    kk::BpAngle rec;
+    const kk::BpAngle& P =
+        TABLE_IN_MEMORY ? (rec = kk_uniform_load(tab + a), rec) : tab[a];
     const double den = fma(Y, P.dy, fma(X, P.dx, P.dw));
     const double nu = fma(Y, P.nuy, fma(X, P.nux, P.nuw));
     const double nv = fma(Y, P.nvy, fma(X, P.nvx, P.nvw));
@@ -348,7 +352,8 @@ constexpr int bp_min_blocks(int k) { return k <= 8 ? 5 : 3; }
 template <int K>
 __global__ __launch_bounds__(256, bp_min_blocks(K)) void cone_bp_kernel(
     const kk::BpAngle* __restrict__ tab, BpArgs A) {
-  bp_tile<K>(tab, A, kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
+  bp_tile<K, false>(tab, A,
+                    kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
 }
 
 struct BpItem {
@@ -372,7 +377,7 @@ __global__ __launch_bounds__(256, bp_min_blocks(K)) void cone_bp_batch_kernel(
     }
     const BpItem& it = items[lo];
     const int64_t local = g - first[lo];
-    bp_tile<K>(it.tab, it.args,
+    bp_tile<K, true>(it.tab, it.args,
                kk_xcd_remap(local, it.args.total_blocks, it.args.xcd_group));
   }
 }

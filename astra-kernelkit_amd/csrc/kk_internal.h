@@ -198,10 +198,30 @@ __device__ __forceinline__ float kk_clamp(float x, float lo, float hi) {
 // Loads at a wave-uniform base plus a 32-bit per-lane byte offset: the
 // compiler emits the SGPR-base ("saddr") form of global loads, so no 64-bit
 // address arithmetic is spent per lane.
-// Four consecutive floats, 8-byte aligned: one global_load_dwordx4.
+// Four consecutive floats, 8-byte aligned: one global_load_dwordx4.  The
+// address is stated to be global memory (textures are device allocations):
+// for a base read from a descriptor in memory (the batch kernels) the
+// compiler would otherwise fall back to flat loads, which cost more and
+// also count against the LDS/scalar wait counter.
 typedef float kk_float4u __attribute__((ext_vector_type(4), aligned(8)));
+typedef const __attribute__((address_space(1))) kk_float4u* kk_gptr4;
 __device__ __forceinline__ kk_float4u kk_ld4(const char* base, uint32_t off) {
-  return *reinterpret_cast<const kk_float4u*>(base + off);
+  return *(kk_gptr4)(base + off);
+}
+
+// Copy of a wave-uniform, read-only record (an angle's parameters) read as
+// constant memory, so it always comes through scalar loads into SGPRs,
+// also when the table pointer was itself read from memory.
+template <class T>
+__device__ __forceinline__ T kk_uniform_load(const T* p) {
+  static_assert(sizeof(T) % 16 == 0, "record of whole 16-byte words");
+  typedef uint32_t word4 __attribute__((ext_vector_type(4)));
+  typedef const __attribute__((address_space(4))) word4* cptr;
+  const cptr q = (cptr)(p);
+  word4 w[sizeof(T) / 16];
+#pragma unroll
+  for (size_t i = 0; i < sizeof(T) / 16; ++i) w[i] = q[i];
+  return __builtin_bit_cast(T, w);
 }
 
 __device__ __forceinline__ int32_t kk_wave_min(int32_t v) {
