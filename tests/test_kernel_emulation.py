@@ -598,3 +598,49 @@ def test_filter_radix4_index_arithmetic(N):
     got = _emulate_filter_fft(x, he[rev] / N)
     ref = np.fft.ifft(np.fft.fft(x) * he)
     np.testing.assert_allclose(got, ref, rtol=0, atol=1e-12 * np.abs(x).max())
+
+
+def _xcd_remap(bid, total, group, n_xcd=8):
+    """kk_xcd_remap of kk_internal.h on an array of hardware ids."""
+    bid = np.asarray(bid, np.int64)
+    if group <= 1:
+        return bid.copy()
+    span = n_xcd * group
+    chunk = bid // span
+    r = bid - chunk * span
+    out = chunk * span + (r % n_xcd) * group + r // n_xcd
+    return np.where((chunk + 1) * span > total, bid, out)
+
+
+@pytest.mark.parametrize("total,group", [(1, 64), (511, 64), (512, 64),
+                                         (513, 64), (1000003, 64),
+                                         (777, 30), (4096, 8), (90, 1)])
+def test_xcd_remap_is_a_bijection_with_runs_per_xcd(total, group):
+    """Every logical tile is run exactly once, and inside full chunks the
+    ids dealt to one XCD (hardware id mod 8) are `group` consecutive
+    logical ids."""
+    hw = np.arange(total, dtype=np.int64)
+    logical = _xcd_remap(hw, total, group)
+    assert np.array_equal(np.sort(logical), hw)
+    span = 8 * group
+    full = (total // span) * span if group > 1 else 0
+    for c0 in range(0, min(full, 4 * span), span):
+        for x in range(8):
+            ids = logical[c0 + x:c0 + span:8]
+            assert np.array_equal(ids, c0 + x * group + np.arange(group))
+
+
+def test_fp_tile_decode_covers_every_tile_once():
+    """The forward kernel's logical id -> (angle, column tile, row band)
+    decode (angle fastest) composed with the XCD remap visits every tile
+    once, for the 512^3 / 720-angle launch shape (16-column workgroups,
+    128-row bands)."""
+    n_ang, col_tiles, row_tiles = 720, 48, 6
+    total = n_ang * col_tiles * row_tiles
+    bid = _xcd_remap(np.arange(total, dtype=np.int64), total, 64)
+    ang = bid % n_ang
+    t = bid // n_ang
+    ct, rt = t % col_tiles, t // col_tiles
+    key = (rt * col_tiles + ct) * n_ang + ang
+    assert np.array_equal(np.sort(key), np.arange(total))
+    assert rt.max() == row_tiles - 1 and ct.max() == col_tiles - 1
