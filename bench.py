@@ -18,10 +18,13 @@ The JSON line also carries:
   aggregate L2 read rate 8 XCD x 16 channels x 128 B/clk x 2.4 GHz;
   ``traffic`` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE +
   WRITE_SIZE counters (scaled by a plain-copy calibration), read from
-  ``profiles/cone_fp_bp_512cube_720_traffic.json`` when present;
-* ``cpu_baseline``: the FP64 NumPy oracle on one host core, timed on a
-  bounded sample of the same workload (one angle, a band of detector rows
-  for FP and a slab of the volume for BP), in GUPS.
+  ``profiles/cone_fp_bp_512cube_720_traffic.json`` when present, and
+  ``traffic_GBs`` / ``traffic_frac_of_hbm`` that amount at this run's
+  kernel time against the 8 TB/s HBM peak;
+* ``cpu_baseline``: the FP64 NumPy oracle in 16 host processes of one core
+  each (``single_core``: one process), timed on a bounded sample of the
+  same workload (one angle, a band of detector rows for FP and a slab of
+  the volume for BP), in GUPS.
 """
 from __future__ import annotations
 
@@ -634,6 +637,13 @@ def main(argv=None):
             "l2_measured_GBs": round(l2_rate, 1),
             "frac_of_measured_l2": round(achieved / l2_rate, 4),
             "hbm_peak": HBM_PEAK_GBS,
+            # measured beyond-L2 (fabric/HBM) bytes of the launch at this
+            # launch's time, against the HBM peak
+            "traffic_GBs": (None if traffic is None else
+                            round(traffic / (t_fp * 1e-3) / 1e9, 1)),
+            "traffic_frac_of_hbm": (
+                None if traffic is None else
+                round(traffic / (t_fp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
             "bp_kernel": {
                 "achieved": round(bp_bytes / (t_bp * 1e-3) / 1e9, 1),
                 "frac": round(bp_bytes / (t_bp * 1e-3) / 1e9 / L2_PEAK_GBS,
