@@ -471,7 +471,10 @@ int make_bp_args(const kk::BpSpec& S, int32_t zrun, BpArgs& A) {
   if (A.total_blocks >= (int64_t(1) << 31))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: grid too large");
   const int32_t plane = A.tiles_x * A.tiles_y;
-  A.xcd_group = 1;
+  // runs of 16 consecutive tiles per XCD (measured at 512^3 / 720 angles:
+  // 1.5% faster than round-robin, 4% faster than one contiguous part of
+  // the (x, y) plane per XCD)
+  A.xcd_group = 16;
   return KK_OK;
 }
 
