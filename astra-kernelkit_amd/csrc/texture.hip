@@ -36,9 +36,19 @@ __global__ __launch_bounds__(256) void fp_frame_kernel(
 
 // Projection texture: element (a, jv, ju) of a [n][rows+3][cols+3] float2
 // array = (P(a, jv-1, ju-1), P(a, jv, ju-1)), zero outside.
+// RESID: the texture of (proj - y) * R instead (the SIRT residual), with y
+// and R addressed by their own strides.
+struct ResidArgs {
+  const float* y;
+  int64_t y0, y1, y2;
+  const float* R;
+  int64_t r0, r1, r2;
+};
+
+template <bool RESID>
 __global__ __launch_bounds__(256) void bp_tex_kernel(
     const float* __restrict__ proj, int64_t s0, int64_t s1, int64_t s2,
-    int32_t rows, int32_t cols, float2* __restrict__ tex) {
+    int32_t rows, int32_t cols, float2* __restrict__ tex, ResidArgs res) {
   const int32_t tc = cols + kk::kHalo;
   const int32_t tr = rows + kk::kHalo;
   const int32_t ju = blockIdx.x * 256 + threadIdx.x;
@@ -47,11 +57,17 @@ __global__ __launch_bounds__(256) void bp_tex_kernel(
   if (ju >= tc) return;
   const int32_t c = ju - kk::kHaloLo;
   const int32_t r0 = jv - kk::kHaloLo, r1 = r0 + 1;
+  auto value = [&](int32_t r) {
+    float v = proj[a * s0 + r * s1 + c * s2];
+    if constexpr (RESID)
+      v = (v - res.y[a * res.y0 + r * res.y1 + c * res.y2]) *
+          res.R[a * res.r0 + r * res.r1 + c * res.r2];
+    return v;
+  };
   float v0 = 0.f, v1 = 0.f;
   if (c >= 0 && c < cols) {
-    const float* p = proj + a * s0 + c * s2;
-    if (r0 >= 0 && r0 < rows) v0 = p[r0 * s1];
-    if (r1 >= 0 && r1 < rows) v1 = p[r1 * s1];
+    if (r0 >= 0 && r0 < rows) v0 = value(r0);
+    if (r1 >= 0 && r1 < rows) v1 = value(r1);
   }
   tex[(int64_t(a) * tr + jv) * tc + ju] = make_float2(v0, v1);
 }
@@ -107,9 +123,33 @@ extern "C" int kk_projections_to_bp_texture(const float* proj, int32_t n,
                     "kk_projections_to_bp_texture: too large");
   const dim3 grid(unsigned((cols + kk::kHalo + 255) / 256),
                   unsigned(rows + kk::kHalo), unsigned(n));
-  hipLaunchKernelGGL(bp_tex_kernel, grid, dim3(256), 0,
+  hipLaunchKernelGGL(bp_tex_kernel<false>, grid, dim3(256), 0,
                      static_cast<hipStream_t>(stream), proj, st[0], st[1],
-                     st[2], rows, cols, reinterpret_cast<float2*>(tex));
+                     st[2], rows, cols, reinterpret_cast<float2*>(tex),
+                     ResidArgs{});
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return kk::hip_fail(e, "bp_tex_kernel");
+  return KK_OK;
+}
+
+extern "C" int kk_residual_to_bp_texture(
+    const float* proj, const int64_t ps[3], const float* y,
+    const int64_t ys[3], const float* R, const int64_t rs[3],
+    int32_t n_angles, int32_t rows, int32_t cols, float* tex, void* stream) {
+  if (proj == nullptr || ps == nullptr || y == nullptr || ys == nullptr ||
+      R == nullptr || rs == nullptr || tex == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_residual_to_bp_texture: null");
+  if (n_angles <= 0 || rows <= 0 || cols <= 0)
+    return kk::fail(KK_ERR_INVALID, "kk_residual_to_bp_texture: bad sizes");
+  if (n_angles > 65535 || rows + kk::kHalo > 65535)
+    return kk::fail(KK_ERR_UNSUPPORTED,
+                    "kk_residual_to_bp_texture: too large");
+  const dim3 grid(unsigned((cols + kk::kHalo + 255) / 256),
+                  unsigned(rows + kk::kHalo), unsigned(n_angles));
+  const ResidArgs res{y, ys[0], ys[1], ys[2], R, rs[0], rs[1], rs[2]};
+  hipLaunchKernelGGL(bp_tex_kernel<true>, grid, dim3(256), 0,
+                     static_cast<hipStream_t>(stream), proj, ps[0], ps[1],
+                     ps[2], rows, cols, reinterpret_cast<float2*>(tex), res);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kk::hip_fail(e, "bp_tex_kernel");
   return KK_OK;

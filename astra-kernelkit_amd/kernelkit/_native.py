@@ -101,6 +101,10 @@ _SIGNATURES = {
                                            _c_i64p, _vp, ctypes.c_int32,
                                            ctypes.c_int32, ctypes.c_int32,
                                            _vp]),
+    "kk_residual_to_bp_texture": (ctypes.c_int, [_vp, _c_i64p, _vp, _c_i64p,
+                                                 _vp, _c_i64p, ctypes.c_int32,
+                                                 ctypes.c_int32,
+                                                 ctypes.c_int32, _vp, _vp]),
     "kk_sirt_bp_update": (ctypes.c_int, [_vp, _vp, ctypes.c_int32,
                                          ctypes.c_int32, ctypes.c_int32, _vp,
                                          _c_i32p, _c_i64p, _vp, _c_i64p, _vp,

@@ -185,16 +185,27 @@ class _SirtState:
         self.clip_lo = 0 if min_constraint is None else 1
         self.clip_hi = 0 if max_constraint is None else 1
         self.scale = float(volume_geometry.voxel_volume())
+        # forward-projection buffer of the residual step
+        self.p = torch.empty((n, rows, cols), dtype=torch.float32,
+                             device=dev)
 
     def iterate(self):
         lib = _native.lib()
         st = _native.stream_ptr()
         fk, bk = self.fptor.kernel, self.bptor.kernel
-        _native.check(lib.kk_sirt_fp_residual(
+        # residual (FP(x) - y) * R into the BP texture: a plain forward
+        # projection into a buffer, then one coalesced pass (cheaper on
+        # MI355X than the forward kernel's residual epilogue, which reads
+        # y and R one detector row per lane)
+        _native.check(lib.kk_cone_fp(
             fk._plan.ptr, _native._vp(self.xtex.ptr),
+            _native._vp(self.p.data_ptr()), _native.i64(self.p.stride()),
+            self.rows, self.cols, 0, st))
+        _native.check(lib.kk_residual_to_bp_texture(
+            _native._vp(self.p.data_ptr()), _native.i64(self.p.stride()),
             _native._vp(self.y.data_ptr()), _native.i64(self.y.stride()),
             _native._vp(self.R.data_ptr()), _native.i64(self.R.stride()),
-            _native._vp(self.rtex.ptr), self.n, self.rows, self.cols, st))
+            self.n, self.rows, self.cols, _native._vp(self.rtex.ptr), st))
         mask_ptr = None if self.mask is None else _native._vp(
             self.mask.data_ptr())
         mask_st = None if self.mask is None else _native.i64(

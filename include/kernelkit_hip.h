@@ -215,6 +215,18 @@ int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
                         float* res_tex, int32_t n_angles, int32_t rows,
                         int32_t cols, void* stream);
 
+/* res = (proj - y) * R written as the BP texture res_tex (halo included):
+ * the same residual step of a SIRT iteration (algo.py:339-344) when the
+ * forward projection went to a plain buffer proj first (kk_cone_fp).  One
+ * coalesced pass; proj, y, R: (angle, row, col) element strides.  This is
+ * what algo.sirt uses per problem: the fused epilogue above reads y and R
+ * one detector row per lane, which costs more than this pass. */
+int kk_residual_to_bp_texture(const float* proj, const int64_t proj_strides[3],
+                              const float* y, const int64_t y_strides[3],
+                              const float* R, const int64_t R_strides[3],
+                              int32_t n_angles, int32_t rows, int32_t cols,
+                              float* res_tex, void* stream);
+
 /* x = clip(mask * (x - C * BP(res))) (algo.py:356-362), fused into the
  * backprojection epilogue.  mask (nullable), C, x: world strides; clip_lo /
  * clip_hi select the min/max constraints.  x_tex (nullable): the FP texture
