@@ -179,7 +179,7 @@ class BatchSirt:
             batch._check(masks, batch.domain_shapes, "mask")
         else:
             masks = [None] * len(batch)
-        self.y, self.x, self.C, self.R = ys, [], [], []
+        self.y, self.x, self.C, self.R, self.p = ys, [], [], [], []
         self.masks = masks
         self.res_tex = []
         self._xtex = []
@@ -201,11 +201,15 @@ class BatchSirt:
             rtex.zero()
             fk = op.projector.kernel
             bk = op.backprojector.kernel
-            _native.check(lib.kk_batch_add_sirt_fp(
+            # residual in two steps: the batch FP into a buffer, then one
+            # coalesced (p - y) R pass per problem into its BP texture (the
+            # fused epilogue, kk_batch_add_sirt_fp, reads y and R one
+            # detector row per lane, which costs more on MI355X)
+            p = torch.empty((n, rows, cols), dtype=torch.float32, device=dev)
+            _native.check(lib.kk_batch_add_fp(
                 b.ptr, fk._plan.ptr, _native._vp(xtex.ptr),
-                _native._vp(y.data_ptr()), _native.i64(y.stride()),
-                _native._vp(R.data_ptr()), _native.i64(R.stride()),
-                _native._vp(rtex.ptr), n, rows, cols))
+                _native._vp(p.data_ptr()), _native.i64(p.stride()), rows,
+                cols, 0))
             _native.check(lib.kk_batch_add_sirt_bp(
                 b.ptr, bk._plan.ptr, _native._vp(rtex.ptr), n, rows, cols,
                 _native._vp(x.data_ptr()), _native.i32(shape),
@@ -222,6 +226,7 @@ class BatchSirt:
             self.x.append(x)
             self.C.append(C)
             self.R.append(R)
+            self.p.append(p)
             self.res_tex.append(rtex)
             self._xtex.append(xtex)
         self.handle = b
@@ -244,14 +249,22 @@ class BatchSirt:
 
     def iterate(self):
         """One SIRT iteration of every problem: one persistent launch per
-        operator, then the refresh of each x's forward-projection frames
-        (coalesced texture kernels; the update epilogue's scattered frame
-        stores cost several times more)."""
+        operator with the residual pass of each problem between them, then
+        the refresh of each x's forward-projection frames (coalesced
+        texture kernels; the update epilogue's scattered frame stores cost
+        several times more)."""
         if self.cfg_fp is None:
             self.configure(tune=False)
         lib = _native.lib()
         st = _native.stream_ptr()
         _native.check(lib.kk_batch_fp(self.handle.ptr, self.cfg_fp.grid, st))
+        for p, y, R, t in zip(self.p, self.y, self.R, self.res_tex):
+            n, rows, cols = p.shape
+            _native.check(lib.kk_residual_to_bp_texture(
+                _native._vp(p.data_ptr()), _native.i64(p.stride()),
+                _native._vp(y.data_ptr()), _native.i64(y.stride()),
+                _native._vp(R.data_ptr()), _native.i64(R.stride()),
+                n, rows, cols, _native._vp(t.ptr), st))
         _native.check(lib.kk_batch_bp(self.handle.ptr, self.cfg_bp.grid, st))
         for t, x in zip(self._xtex, self.x):
             t.update(x)

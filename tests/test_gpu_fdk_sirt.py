@@ -140,3 +140,57 @@ def test_sirt_residual_fixed_point(device):
     kk.sirt(_host(y), pgs, vg, iters=8, verbose=False,
             callback=lambda i, x, r: norms.append(float(r.norm())))
     assert all(b <= a * 1.0001 for a, b in zip(norms, norms[1:]))
+
+
+@pytest.mark.parametrize("pert", [{}, {"wobble_deg": 1.0, "drift_px": 2.0}])
+def test_fused_residual_equals_two_step(pert, device):
+    """The fused forward-projection residual epilogue (kk_sirt_fp_residual,
+    and the batch kernel's kk_batch_add_sirt_fp) writes exactly the BP
+    texture of the two-step form algo.sirt uses (plain kk_cone_fp into a
+    buffer, then kk_residual_to_bp_texture): the same float operations."""
+    import torch
+    from kernelkit import _native
+    from kernelkit.kernel import ProjectionTexture
+    pgs, vg, vol = _case(**pert)
+    op = kk.XrayTransform(pgs, vg)
+    x = _dev(vol)
+    n, rows, cols = len(pgs), pgs[0].detector.rows, pgs[0].detector.cols
+    rng = np.random.default_rng(5)
+    y = _dev(rng.uniform(0.0, 5.0, (n, rows, cols)))
+    R = _dev(rng.uniform(0.1, 2.0, (n, rows, cols)))
+    op.projector.volume = x
+    xtex = op.projector._texture
+    fk = op.projector.kernel
+    lib = _native.lib()
+    st = _native.stream_ptr()
+    texs = [ProjectionTexture(n, rows, cols, device=x.device)
+            for _ in range(3)]
+    for t in texs:
+        t.zero()
+    _native.check(lib.kk_sirt_fp_residual(
+        fk._plan.ptr, _native._vp(xtex.ptr), _native._vp(y.data_ptr()),
+        _native.i64(y.stride()), _native._vp(R.data_ptr()),
+        _native.i64(R.stride()), _native._vp(texs[0].ptr), n, rows, cols,
+        st))
+    p = torch.empty((n, rows, cols), dtype=torch.float32, device=x.device)
+    _native.check(lib.kk_cone_fp(
+        fk._plan.ptr, _native._vp(xtex.ptr), _native._vp(p.data_ptr()),
+        _native.i64(p.stride()), rows, cols, 0, st))
+    _native.check(lib.kk_residual_to_bp_texture(
+        _native._vp(p.data_ptr()), _native.i64(p.stride()),
+        _native._vp(y.data_ptr()), _native.i64(y.stride()),
+        _native._vp(R.data_ptr()), _native.i64(R.stride()), n, rows, cols,
+        _native._vp(texs[1].ptr), st))
+    b = _native.Batch()
+    _native.check(lib.kk_batch_add_sirt_fp(
+        b.ptr, fk._plan.ptr, _native._vp(xtex.ptr),
+        _native._vp(y.data_ptr()), _native.i64(y.stride()),
+        _native._vp(R.data_ptr()), _native.i64(R.stride()),
+        _native._vp(texs[2].ptr), n, rows, cols))
+    _native.check(lib.kk_batch_configure(b.ptr, 0, 8))
+    _native.check(lib.kk_batch_fp(b.ptr, 0, st))
+    torch.cuda.synchronize()
+    ref = (_host(p) - _host(y)) * _host(R)
+    assert float(np.abs(ref).max()) > 0
+    assert torch.equal(texs[0].data, texs[1].data)
+    assert torch.equal(texs[2].data, texs[1].data)
