@@ -156,3 +156,20 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setenv("KK_LIBRARY", str(tmp_path / "missing.so"))
     with pytest.raises(_native.NativeLibraryError):
         _native.lib()
+
+
+def test_residual_texture_validation(lib):
+    """kk_residual_to_bp_texture rejects null arguments and bad sizes on
+    the host, before any launch."""
+    st = _native.i64((1, 1, 1))
+    buf = ctypes.c_void_p(16)   # never dereferenced: validation fails first
+    rc = lib.kk_residual_to_bp_texture(None, st, buf, st, buf, st, 1, 1, 1,
+                                       buf, None)
+    assert rc == _native.KK_ERR_INVALID
+    rc = lib.kk_residual_to_bp_texture(buf, st, buf, st, buf, st, 0, 4, 4,
+                                       buf, None)
+    assert rc == _native.KK_ERR_INVALID
+    assert "bad sizes" in _native.last_error()
+    rc = lib.kk_residual_to_bp_texture(buf, st, buf, st, buf, st, 70000, 4,
+                                       4, buf, None)
+    assert rc == _native.KK_ERR_UNSUPPORTED
