@@ -16,11 +16,10 @@
 //   is: one FMA per coordinate, a clamp into the halo, v_cvt_u32 + v_fract
 //   for the tap index and weight (the slice index itself is converted to
 //   float64 with scalar integer instructions), and ONE 16-byte load for
-//   all four taps
-//   (a 64-lane gather instruction costs the texture-address unit ~21
-//   cycles for 8 or 16 bytes per lane, measured on MI355X, about as many
-//   as the ~20 VALU instructions of a sample: the kernel is balanced
-//   between the two);
+//   all four taps (~15 VALU instructions per sample in the unclamped
+//   interior; measured on MI355X at 512^3 / 720 angles the kernel keeps
+//   the texture-address units ~85% busy, ~14 cycles per 64-lane gather,
+//   and the VALU about as busy: it is close to both issue limits);
 // * sample coordinates are evaluated directly, x_b(i) = a1*i + beta, instead
 //   of the reference's running sums, so the per-sample error does not grow
 //   along the ray;
