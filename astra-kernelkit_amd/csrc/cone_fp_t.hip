@@ -20,8 +20,9 @@
 //    float atomic add costs ~190 cycles per wavefront instruction on this
 //    part, against ~8 for a plain LDS read-modify-write).  A lane owns kJB x kJC
 //    voxels (kJB along b, kJC consecutive along c) of one slice, the 64
-//    lanes of a wavefront consecutive c pairs, accumulated in registers
-//    over all angles of the launch's ray axis and written once.  Owning
+//    lanes of a wavefront consecutive c pairs, accumulated in the lane's
+//    own LDS words over all angles of the launch's ray axis and written
+//    once.  Owning
 //    two c voxels, a lane evaluates each of its samples once for both
 //    (a sample has two c taps): 3/2 evaluations per sample instead of 2.  Per angle:
 //    * circular orbits ("column-coherent" angles: the row step has no a/b
@@ -31,7 +32,8 @@
 //      compute 64 columns' ray data at a time and broadcast them with
 //      readlane (scalar registers); per column, the c coordinate is affine
 //      in the row, so each lane sums the rows reaching its voxel and the
-//      sum enters the b-run with the column's two b taps (wave-uniform);
+//      sum enters the b-run with the column's two b taps (a wave-uniform
+//      LDS row);
 //    * other geometries: each lane projects the support rectangle of its
 //      voxels (at its slice) onto the detector and walks the pixels of the
 //      bounding box; the b taps, whose index differs per lane, are summed
@@ -165,23 +167,6 @@ __device__ __forceinline__ double bcast(double v, int32_t l) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// acc[t0][cc] += (1 - fb) s[cc], acc[t0 + 1][cc] += fb s[cc] for a
-// wave-uniform t0 in [-1, kJB): scalar branches, 2 kJC FMAs.
-__device__ __forceinline__ void add_b_taps(float (&acc)[kJB][kJC],
-                                           int32_t t0, float fb,
-                                           const float (&s)[kJC]) {
-#pragma unroll
-  for (int t = -1; t < kJB; ++t) {
-    if (t == t0) {
-#pragma unroll
-      for (int cc = 0; cc < kJC; ++cc) {
-        if (t >= 0) acc[t][cc] = fmaf(1.f - fb, s[cc], acc[t][cc]);
-        if (t + 1 < kJB) acc[t + 1][cc] = fmaf(fb, s[cc], acc[t + 1][cc]);
-      }
-    }
-  }
-}
-
 // Sample of row r in a column-coherent column (D0c, inv: the column's
 // kk_fp_ray terms): adds y times its c-tap weight to the lane's kJC
 // column sums (voxels k .. k + kJC - 1).
@@ -219,14 +204,11 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
   const double kd = static_cast<double>(k);
   const float kf = static_cast<float>(k);
   const int64_t angle_elems = static_cast<int64_t>(A.rows) * A.cols;
-  float acc[kJB][kJC];
-#pragma unroll
-  for (int t = 0; t < kJB; ++t)
-#pragma unroll
-    for (int cc = 0; cc < kJC; ++cc) acc[t][cc] = 0.f;
-  // general path: b-tap sums of this lane at t = -1 .. kJB (row t + 1),
-  // indexed per lane; every lane owns its words, so no atomics and no
-  // cross-lane ordering are needed
+  // b-tap sums of this lane's voxels at t = -1 .. kJB (row t + 1; rows 0
+  // and kJB + 1 catch taps just outside the b-run and are discarded),
+  // indexed per lane: every lane owns its words, so no atomics and no
+  // cross-lane ordering are needed.  (A register array indexed by the
+  // wave-uniform tap row was compiled into private scratch memory.)
   __shared__ float gacc[kWaves][kJB + 2][kJC][64];
   float(*g)[kJC][64] = gacc[threadIdx.x >> 6];
 #pragma unroll
@@ -312,7 +294,15 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
           }
           for (; r <= r_hi; ++r)
             add_c_taps(P, D0cm, invm, fi, nc1, r, k, ycol[r], colsum);
-          add_b_taps(acc, tm, bcast(fb, m), colsum);
+          // the column's two b taps (tm wave-uniform): read-modify-write
+          // of the lane's own LDS words
+          const float fbm = bcast(fb, m);
+#pragma unroll
+          for (int cc = 0; cc < kJC; ++cc) {
+            g[tm + 1][cc][lane] = fmaf(1.f - fbm, colsum[cc],
+                                       g[tm + 1][cc][lane]);
+            g[tm + 2][cc][lane] = fmaf(fbm, colsum[cc], g[tm + 2][cc][lane]);
+          }
         }
       }
     } else {
@@ -412,7 +402,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
     for (int t = 0; t < kJB; ++t) {
       const int32_t j = jw + t;
       if (j >= A.nb) break;
-      const float v = acc[t][cc] + g[t + 1][cc][lane];
+      const float v = g[t + 1][cc][lane];
       const int64_t o = i * A.sa + j * A.sb + kc * A.sc;
       if (A.mode == 1)
         A.vol[o] += v;
