@@ -644,3 +644,29 @@ def test_fp_tile_decode_covers_every_tile_once():
     key = (rt * col_tiles + ct) * n_ang + ang
     assert np.array_equal(np.sort(key), np.arange(total))
     assert rt.max() == row_tiles - 1 and ct.max() == col_tiles - 1
+
+
+def _uniform_i2d_bits(i):
+    """kk_uniform_i2d of kk_internal.h: the float64 bits of int32 i >= 0
+    assembled from the leading-one position (0 maps to +0.0)."""
+    u = np.asarray(i, dtype=np.uint64)
+    clz = np.array([32 - int(v).bit_length() if v else 31 for v in u | 1],
+                   dtype=np.uint64)   # __builtin_clz(u | 1)
+    e = np.uint64(31) - clz
+    bits = (u << (np.uint64(52) - e)) + ((np.uint64(1022) + e) << np.uint64(52))
+    return np.where(u == 0, np.uint64(0), bits)
+
+
+def test_uniform_i2d_matches_int_to_double():
+    """The slice index's scalar float64 construction equals (double)i bit
+    for bit over the int32 range (0, powers of two and their neighbours,
+    the maximum, random values)."""
+    rng = np.random.default_rng(3)
+    p2 = np.array([1 << k for k in range(31)], dtype=np.int64)
+    vals = np.unique(np.concatenate([
+        np.arange(0, 4097), p2, p2 - 1, p2 + 1, [2 ** 31 - 1],
+        rng.integers(0, 2 ** 31, 200000)]))
+    vals = vals[(vals >= 0) & (vals < 2 ** 31)]
+    got = _uniform_i2d_bits(vals.astype(np.uint64))
+    want = vals.astype(np.float64).view(np.uint64)
+    np.testing.assert_array_equal(got, want)
