@@ -414,12 +414,17 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
 
 }  // namespace
 
-extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
-                                    const int64_t ps[3], int32_t rows,
-                                    int32_t cols, float* vol,
-                                    const int32_t vol_shape[3],
-                                    const int64_t vs[3], int32_t additive,
-                                    float* work, void* stream) {
+int64_t kk::fp_transpose_work_floats(const kk_plan_s* plan, int32_t rows,
+                                     int32_t cols) {
+  if (plan == nullptr || rows <= 0 || cols <= 0) return -1;
+  return int64_t(plan->fp_n) * rows * cols;
+}
+
+int kk::fp_transpose_launch(const kk_plan_s* plan, const float* proj,
+                            const int64_t ps[3], int32_t rows, int32_t cols,
+                            float* vol, const int32_t vol_shape[3],
+                            const int64_t vs[3], int32_t additive, float* work,
+                            hipStream_t st) {
   if (plan == nullptr || !plan->fp_set)
     return kk::fail(KK_ERR_NOT_SET,
                     "Please set the projection geometry with `set_params()`.");
@@ -436,7 +441,6 @@ extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
       return kk::fail(KK_ERR_SHAPE,
                       "kk_cone_fp_transpose: volume shape differs from the "
                       "one the parameters were set for");
-  const hipStream_t st = static_cast<hipStream_t>(stream);
   {
     const int32_t tr = (rows + kTile - 1) / kTile;
     const int32_t tcl = (cols + kTile - 1) / kTile;
@@ -480,4 +484,15 @@ extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
     if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_t_kernel");
   }
   return KK_OK;
+}
+
+extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
+                                    const int64_t ps[3], int32_t rows,
+                                    int32_t cols, float* vol,
+                                    const int32_t vol_shape[3],
+                                    const int64_t vs[3], int32_t additive,
+                                    float* work, void* stream) {
+  return kk::fp_transpose_launch(plan, proj, ps, rows, cols, vol, vol_shape,
+                                 vs, additive, work,
+                                 static_cast<hipStream_t>(stream));
 }

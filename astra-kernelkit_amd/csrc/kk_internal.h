@@ -157,6 +157,17 @@ int sirt_residual_spec(FpSpec& S, const kk_plan_s* plan, const float* x_tex,
                        const int64_t rs[3], float* res_tex, int32_t n_angles,
                        int32_t rows, int32_t cols);
 
+// Exact transpose of the forward projection (cone_fp_t.hip): the floats of
+// workspace a launch needs (-1 on bad arguments), and the launch itself
+// with the workspace given (validates, allocates nothing).
+int64_t fp_transpose_work_floats(const kk_plan_s* plan, int32_t rows,
+                                 int32_t cols);
+int fp_transpose_launch(const kk_plan_s* plan, const float* proj,
+                        const int64_t proj_strides[3], int32_t rows,
+                        int32_t cols, float* vol, const int32_t vol_shape[3],
+                        const int64_t vol_strides[3], int32_t additive,
+                        float* work, hipStream_t stream);
+
 }  // namespace kk
 
 struct kk_plan_s {

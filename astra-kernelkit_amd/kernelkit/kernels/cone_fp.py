@@ -329,7 +329,8 @@ class RayDrivenConeFP(BaseKernel):
 
     def transpose(self, projections, projection_geometry,
                   volume_geometry: VolumeGeometry, volume, *,
-                  additive: bool = False, stream=None):
+                  additive: bool = False, stream=None,
+                  allocate: bool = True):
         """Exact transpose of this forward projection (matched adjoint):
         ``volume (+)= A^T projections``.
 
@@ -340,7 +341,12 @@ class RayDrivenConeFP(BaseKernel):
         adds its weighted value to its bilinear taps, with weights and
         coordinates computed by the forward kernel's own functions (float64
         coordinates, bit-identical weights), so ``<A x, y> = <x, A^T y>`` up
-        to rounding.
+        to rounding.  ``allocate=False`` uses the library's static
+        workspace instead of allocating one (for callers that cannot
+        allocate, e.g. under graph capture; limited to
+        ``kk_cone_fp_transpose_static_capacity()`` floats of
+        ``n_angles * rows * cols``, and calls that overlap on one device
+        race on it).
         """
         if not self.is_compiled:
             raise KernelNotCompiledException(
@@ -374,6 +380,13 @@ class RayDrivenConeFP(BaseKernel):
         self._upload(rows, cols, volume_geometry)
         pst = _logical_strides(projections, self._proj_axs)
         vst = _logical_strides(volume, self._vol_axs)
+        if not allocate:
+            _native.check(_native.lib().kk_cone_fp_transpose_static(
+                self._plan.ptr, _native._vp(projections.data_ptr()),
+                _native.i64(pst), rows, cols, _native._vp(volume.data_ptr()),
+                _native.i32(world), _native.i64(vst), 1 if additive else 0,
+                _native.stream_ptr(stream)))
+            return
         # the library's scratch: weighted projections, column-major
         work = torch.empty(n * rows * cols, dtype=torch.float32,
                            device=projections.device)

@@ -144,6 +144,26 @@ int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
                          const int64_t vol_strides[3], int32_t additive,
                          float* work, void* stream);
 
+/* kk_cone_fp_transpose without the caller workspace: it allocates nothing
+ * (no hipMalloc, no library allocation).  No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * The workspace is a static device array of
+ * kk_cone_fp_transpose_static_capacity() floats (64 Mi floats, 256 MiB) that
+ * every device the library runs on holds until the process ends; problems
+ * with n_angles * rows * cols above it are rejected with
+ * KK_ERR_UNSUPPORTED.  All calls on one device share that array: calls that
+ * overlap on one device (other streams or host threads) race.  Arguments,
+ * results and errors are otherwise those of kk_cone_fp_transpose. */
+int kk_cone_fp_transpose_static(const kk_plan* plan, const float* proj,
+                                const int64_t proj_strides[3], int32_t rows,
+                                int32_t cols, float* vol,
+                                const int32_t vol_shape[3],
+                                const int64_t vol_strides[3], int32_t additive,
+                                void* stream);
+/* Floats of kk_cone_fp_transpose_static's workspace (host-only). */
+int64_t kk_cone_fp_transpose_static_capacity(void);
+
 /* ---- batches of independent problems (one launch per operator) ------ */
 
 /* No counterpart in the reference, which runs one geometry per projector
