@@ -65,6 +65,10 @@ constexpr int32_t kXcdRun = 64;
 struct FpArgs {
   const char* tex;  // frame textures (byte base)
   float* out;
+  // if non-null: the output base is out_ptrs[index along logical axis
+  // ptr_axis] instead of out (whose stride along that axis is then 0)
+  const int64_t* out_ptrs;
+  int32_t ptr_axis;
   int64_t os0, os1, os2;  // output strides: angle, row, col
   int64_t out_dual;       // if nonzero: also write out[o + out_dual]
   const float* y;         // mode 2: measured projections
@@ -342,8 +346,12 @@ __device__ __forceinline__ void fp_tile_at(
     if (row >= A.rows || col >= A.cols) continue;
     float val = acc[k] * wscale[k];
     const int64_t o = ang * A.os0 + row * A.os1 + col * A.os2;
+    float* out = A.out;
+    if (A.out_ptrs != nullptr)
+      out = reinterpret_cast<float*>(
+          A.out_ptrs[A.ptr_axis == 0 ? ang : (A.ptr_axis == 1 ? row : col)]);
     if (A.mode == 1) {
-      A.out[o] += val;
+      out[o] += val;
       continue;
     }
     if (A.mode == 2) {
@@ -351,8 +359,8 @@ __device__ __forceinline__ void fp_tile_at(
       const float rv = A.R[ang * A.rs0 + row * A.rs1 + col * A.rs2];
       val = (val - yv) * rv;
     }
-    A.out[o] = val;
-    if (A.out_dual) A.out[o + A.out_dual] = val;
+    out[o] = val;
+    if (A.out_dual) out[o + A.out_dual] = val;
   }
 }
 
@@ -418,8 +426,13 @@ int make_fp_args(const kk::FpSpec& S, int32_t rpt, int32_t cols_per_block,
   if (S.plan == nullptr || !S.plan->fp_set)
     return fail(KK_ERR_NOT_SET,
                 "Please set the projection geometry with `set_params()`.");
-  if (S.tex == nullptr || S.out == nullptr)
+  if (S.tex == nullptr || (S.out == nullptr && S.ptrs == nullptr))
     return fail(KK_ERR_INVALID, "kk_cone_fp: null argument");
+  if (S.ptrs != nullptr &&
+      (S.mode == 2 || S.out_dual != 0 || S.ptr_axis < 0 || S.ptr_axis > 2))
+    return fail(KK_ERR_INVALID,
+                "kk_cone_fp_ptrs: a pointer table takes plain or additive "
+                "output along axis 0, 1 or 2");
   if (S.rows <= 0 || S.cols <= 0)
     return fail(KK_ERR_INVALID, "kk_cone_fp: bad detector size");
   if (S.mode == 2 && (S.y == nullptr || S.R == nullptr))
@@ -429,9 +442,11 @@ int make_fp_args(const kk::FpSpec& S, int32_t rpt, int32_t cols_per_block,
   A = FpArgs{};
   A.tex = reinterpret_cast<const char*>(S.tex);
   A.out = S.out;
-  A.os0 = S.os[0];
-  A.os1 = S.os[1];
-  A.os2 = S.os[2];
+  A.out_ptrs = S.ptrs;
+  A.ptr_axis = S.ptr_axis;
+  A.os0 = S.ptrs != nullptr && S.ptr_axis == 0 ? 0 : S.os[0];
+  A.os1 = S.ptrs != nullptr && S.ptr_axis == 1 ? 0 : S.os[1];
+  A.os2 = S.ptrs != nullptr && S.ptr_axis == 2 ? 0 : S.os[2];
   A.out_dual = S.out_dual;
   if (S.mode == 2) {
     A.y = S.y;
@@ -683,6 +698,21 @@ extern "C" int kk_cone_fp(const kk_plan* plan, const float* vol_tex,
   const int rc = kk::fp_spec(S, plan, vol_tex, proj, ps, rows, cols,
                              additive);
   if (rc != KK_OK) return rc;
+  return launch_fp_spec(S, 0, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int kk_cone_fp_ptrs(const kk_plan* plan, const float* vol_tex,
+                               const int64_t* ptrs, int32_t ptr_axis,
+                               const int64_t ps[3], int32_t rows,
+                               int32_t cols, int32_t additive, void* stream) {
+  if (ptrs == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_cone_fp_ptrs: null pointer table");
+  kk::FpSpec S;
+  const int rc = kk::fp_spec(S, plan, vol_tex, nullptr, ps, rows, cols,
+                             additive);
+  if (rc != KK_OK) return rc;
+  S.ptrs = ptrs;
+  S.ptr_axis = ptr_axis;
   return launch_fp_spec(S, 0, static_cast<hipStream_t>(stream));
 }
 

@@ -93,6 +93,10 @@ struct FpSpec {
   const kk_plan_s* plan = nullptr;
   const float* tex = nullptr;
   float* out = nullptr;
+  // optional device table of output bases, one per index along logical
+  // axis ptr_axis (0 angle, 1 row, 2 col); then `out` is unused
+  const int64_t* ptrs = nullptr;
+  int32_t ptr_axis = 0;
   int64_t os[3] = {0, 0, 0};
   int64_t out_dual = 0;
   int32_t rows = 0, cols = 0, mode = 0;

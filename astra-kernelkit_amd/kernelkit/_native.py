@@ -68,7 +68,10 @@ _SIGNATURES = {
     "kk_fill": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_float, _vp]),
     "kk_cone_fp": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, ctypes.c_int32,
                                   ctypes.c_int32, ctypes.c_int32, _vp]),
-    "kk_batch_create": (ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
+    "kk_cone_fp_ptrs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32,
+                                       _c_i64p, ctypes.c_int32,
+                                       ctypes.c_int32, ctypes.c_int32, _vp]),
+    "kk_batch_create":(ctypes.c_int, [ctypes.POINTER(ctypes.c_void_p)]),
     "kk_batch_destroy": (ctypes.c_int, [_vp]),
     "kk_batch_size": (ctypes.c_int32, [_vp, ctypes.c_int32]),
     "kk_batch_add_fp": (ctypes.c_int, [
@@ -98,7 +101,7 @@ _SIGNATURES = {
         _vp, _vp, _c_i64p, ctypes.c_int32, ctypes.c_int32, _vp, _c_i32p,
         _c_i64p, ctypes.c_int32, _vp]),
     "kk_cone_fp_transpose_static_capacity": (ctypes.c_int64, []),
-    "kk_cone_bp": (ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
+    "kk_cone_bp":(ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
                                   ctypes.c_int32, _vp, _c_i32p, _c_i64p,
                                   ctypes.c_float, ctypes.c_int32, _vp]),
     "kk_sirt_fp_residual": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, _vp,

@@ -270,34 +270,31 @@ class RayDrivenConeFP(BaseKernel):
         logical = (n, rows, cols)
         expect = tuple(logical[i] for i in self._proj_axs)
         if isinstance(projections, (list, tuple)):
-            import torch
-            if len(projections) != expect[0]:
-                raise ValueError(
-                    "The length of projection array must match the number of "
-                    f"entries along the first axis of `projection_axes` "
-                    f"{self._proj_axs}, {expect[0]}, but got "
-                    f"{len(projections)}.")
-            for p in projections:
-                if not is_device_array(p):
-                    raise TypeError("Projection data must be device tensors, "
-                                    f"but got {type(p)}.")
-                if p.dtype != torch.float32:
-                    raise NotImplementedError(
-                        "Currently there is only support for dtype float32, "
-                        f"but got {p.dtype}.")
-            tmp = torch.empty(expect, dtype=torch.float32,
-                              device=projections[0].device)
-            if additive:
-                for k, p in enumerate(projections):
-                    tmp[k].copy_(p)
-            self._launch(volume_texture, tmp, rows, cols, additive, stream)
-            for k, p in enumerate(projections):
-                p.copy_(tmp[k])
+            table, strides = self._pointer_table(projections, expect)
+            self._launch_ptrs(volume_texture, table, strides, rows, cols,
+                              additive, stream)
             return
         if not is_device_array(projections):
             raise TypeError("Projection data must be a device tensor or a "
                             f"list of them, got {type(projections)}.")
         import torch
+        if projections.dtype == torch.int64:
+            # the reference's form: a device array of entry base addresses
+            # along the first projection axis, with the pitched logical
+            # dimensions (kernels/cone_fp.py:173, :233-243)
+            if projections.dim() != 1 or projections.numel() != expect[0]:
+                raise ValueError(
+                    f"A pointer array needs {expect[0]} entries (the first "
+                    f"axis of `projection_axes` {self._proj_axs}), got shape "
+                    f"{tuple(projections.shape)}.")
+            pitch = logical if projections_pitch is None else tuple(
+                int(p) for p in projections_pitch)
+            strides = [0, 0, 0]
+            strides[self._proj_axs[1]] = pitch[self._proj_axs[2]]
+            strides[self._proj_axs[2]] = 1
+            self._launch_ptrs(volume_texture, projections.contiguous(),
+                              strides, rows, cols, additive, stream)
+            return
         if projections.dtype != torch.float32:
             raise NotImplementedError(
                 "Currently there is only support for dtype float32, but got "
@@ -318,6 +315,56 @@ class RayDrivenConeFP(BaseKernel):
         tex = self.volume_texture(snap.array, vg)
         self._snap_cache = (key, tex)
         return tex
+
+    def _pointer_table(self, projections, expect):
+        """Device table of the entries' base addresses (one per index of
+        the first projection axis, ``projector.py:301-314``) and the
+        (angle, row, col) element strides inside an entry.  The table is
+        cached per list of addresses, so repeated calls (and graph
+        capture) allocate nothing."""
+        import torch
+        if len(projections) != expect[0]:
+            raise ValueError(
+                "The length of projection array must match the number of "
+                f"entries along the first axis of `projection_axes` "
+                f"{self._proj_axs}, {expect[0]}, but got "
+                f"{len(projections)}.")
+        for p in projections:
+            if not is_device_array(p):
+                raise TypeError("Projection data must be device tensors, "
+                                f"but got {type(p)}.")
+            if p.dtype != torch.float32:
+                raise NotImplementedError(
+                    "Currently there is only support for dtype float32, "
+                    f"but got {p.dtype}.")
+            if tuple(p.shape) != expect[1:]:
+                raise ValueError(
+                    f"Projection entry of shape {tuple(p.shape)}, expected "
+                    f"{expect[1:]} (axes {self._proj_axs}).")
+            if tuple(p.stride()) != tuple(projections[0].stride()):
+                raise ValueError("All projection entries need the same "
+                                 "strides (row pitch).")
+        ptrs = tuple(int(p.data_ptr()) for p in projections)
+        if len(set(ptrs)) != len(ptrs):
+            raise ValueError(
+                "All projection files need to reside in own memory.")
+        cached = getattr(self, "_ptr_cache", None)
+        if cached is None or cached[0] != ptrs:
+            table = torch.tensor(ptrs, dtype=torch.int64).to(
+                projections[0].device)
+            self._ptr_cache = (ptrs, table)
+        strides = [0, 0, 0]
+        strides[self._proj_axs[1]] = projections[0].stride(0)
+        strides[self._proj_axs[2]] = projections[0].stride(1)
+        return self._ptr_cache[1], strides
+
+    def _launch_ptrs(self, tex: VolumeTexture, table, strides, rows, cols,
+                     additive, stream):
+        _native.check(_native.lib().kk_cone_fp_ptrs(
+            self._plan.ptr, _native._vp(tex.ptr),
+            _native._vp(table.data_ptr()), self._proj_axs[0],
+            _native.i64(strides), rows, cols, 1 if additive else 0,
+            _native.stream_ptr(stream)))
 
     def _launch(self, tex: VolumeTexture, proj, rows, cols, additive,
                 stream):

@@ -115,6 +115,23 @@ int kk_cone_fp(const kk_plan* plan, const float* vol_tex, float* proj,
                const int64_t proj_strides[3], int32_t rows, int32_t cols,
                int32_t additive, void* stream);
 
+/* kk_cone_fp writing through a table of base pointers: the reference's
+ * `float** projections` argument (kernelkit/cuda/cone_fp.cu:76, :199-204),
+ * an int64 device array with one base address per index of the first
+ * projection array axis (kernels/cone_fp.py:173, built by
+ * projector.py:301-314 from a list of per-angle, per-row or per-column
+ * arrays).  ptrs: device array of int64 addresses; ptr_axis: the logical
+ * axis (0 angle, 1 row, 2 col) the table is indexed by; proj_strides: the
+ * element strides of the other two logical axes inside every entry (the
+ * ptr_axis stride is ignored), e.g. for the reference's pitched entries
+ * stride(projection_axes[1]) = pitch of projection_axes[2] and
+ * stride(projection_axes[2]) = 1.  Plain or additive output as kk_cone_fp.
+ * Allocates nothing: the table stays valid in a captured graph. */
+int kk_cone_fp_ptrs(const kk_plan* plan, const float* vol_tex,
+                    const int64_t* ptrs, int32_t ptr_axis,
+                    const int64_t proj_strides[3], int32_t rows,
+                    int32_t cols, int32_t additive, void* stream);
+
 /* Voxel-driven backprojection, all angles in one launch (cone_bp,
  * kernelkit/cuda/cone_bp.cu:35-159, kernels/cone_bp.py:292-302).
  * proj_tex: BP texture; n_angles must equal the parameter count.  vol:
@@ -223,12 +240,16 @@ int kk_fp_table(const double* params, int32_t n_angles,
                 void* out, int32_t* frame_mask);
 int kk_bp_table(const double* params, int32_t n_angles, void* out);
 
-/* ---- SIRT (kernelkit/algo.py:226-368), fused ------------------------- */
+/* ---- SIRT (kernelkit/algo.py:226-368) ---------------------------------- */
 
 /* res = (FP(x) - y) * R for every ray: the residual step of one SIRT
  * iteration (algo.py:339-344) fused into the forward projection epilogue,
  * written straight into the BP texture res_tex (whose halo must be zero).
- * y, R: (angle, row, col) element strides. */
+ * y, R: (angle, row, col) element strides.  Not on the product path (nor is
+ * its batch form kk_batch_add_sirt_fp): the SIRT loops use kk_cone_fp plus
+ * kk_residual_to_bp_texture, which is faster on MI355X; it is kept, and
+ * tested to write the same texture bit for bit, for callers that want one
+ * launch instead of two. */
 int kk_sirt_fp_residual(const kk_plan* plan, const float* x_tex,
                         const float* y, const int64_t y_strides[3],
                         const float* R, const int64_t R_strides[3],
