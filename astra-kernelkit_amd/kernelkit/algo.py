@@ -5,11 +5,12 @@ are copied to the device; results come back as host numpy arrays unless the
 caller passes device outputs (``out``) or asks for them (``return_gpu``).
 
 ``fdk`` fuses pre-weight + row filter into one kernel that writes straight
-into the backprojector's texture.  ``sirt`` runs two launches per
-iteration: the forward projection writes ``R * (A x - y)`` into the
-backprojection texture, and the backprojection's epilogue applies
-``x <- clip(mask * (x - C * A^T r))`` and refreshes the forward projector's
-texture of ``x``.
+into the backprojector's texture.  ``sirt`` runs four launches per
+iteration: a plain forward projection of ``x`` into a projection buffer,
+one coalesced pass writing ``R * (A x - y)`` into the backprojection
+texture, the backprojection whose epilogue applies
+``x <- clip(mask * (x - C * A^T r))``, and the rebuild of the forward
+projector's texture of ``x``.
 """
 from __future__ import annotations
 
@@ -125,12 +126,33 @@ def fdk(projections: Any, projection_geometry: Any,
               filter=filter, **bp_kwargs)
 
 
+def _check_mask_shape(mask, shape) -> None:
+    """Raise ``ValueError`` unless ``mask`` broadcasts to the volume shape.
+
+    The reference multiplies ``x *= mask`` (``algo.py:356-357``), so any
+    mask that broadcasts against the volume is accepted; the update kernel
+    reads it through strides (0 along broadcast axes).  Checked on the
+    shape alone, before any device work.
+    """
+    mshape = tuple(int(s) for s in np.shape(mask))
+    try:
+        full = np.broadcast_shapes(mshape, tuple(shape))
+    except ValueError:
+        full = None
+    if full != tuple(shape):
+        raise ValueError(f"mask of shape {mshape} does not broadcast to the "
+                         f"volume shape {tuple(shape)}")
+
+
 class _SirtState:
     """Device buffers and launch arguments of one SIRT problem."""
 
     def __init__(self, projections, projection_geometry, volume_geometry,
                  mask, min_constraint, max_constraint):
         import torch
+        if mask is not None:
+            _check_mask_shape(mask, tuple(int(s) for s in
+                                          volume_geometry.shape))
         self.pg = list(projection_geometry)
         self.vg = volume_geometry
         n = len(self.pg)
@@ -179,7 +201,9 @@ class _SirtState:
         self.rtex = self.bptor._textures
         self.rtex.zero()
         self.res = self.rtex.values()
-        self.mask = None if mask is None else as_device_array(mask)
+        # a broadcast view: stride 0 along the mask's broadcast axes
+        self.mask = (None if mask is None else torch.broadcast_to(
+            as_device_array(mask), shape))
         self.lo = 0.0 if min_constraint is None else float(min_constraint)
         self.hi = 0.0 if max_constraint is None else float(max_constraint)
         self.clip_lo = 0 if min_constraint is None else 1
@@ -223,32 +247,61 @@ class _SirtState:
             None, 0, self.scale, st))
         self.xtex.update(self.x)
 
+    def capture(self):
+        """Record one iteration in a HIP graph (``torch.cuda.graph``): its
+        four launches replay as one.  Nothing is allocated or cleared inside
+        the iteration (every launch writes all it reads next), so the graph
+        holds kernel nodes only."""
+        import torch
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream()
+        s.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.graph(g, stream=s):
+            self.iterate()
+        torch.cuda.current_stream().wait_stream(s)
+        self.graph = g
+        return g
+
+    def step(self):
+        """One iteration: a graph replay once captured, else eager."""
+        g = getattr(self, "graph", None)
+        if g is None:
+            self.iterate()
+        else:
+            g.replay()
+
 
 def sirt(projections: Any, projection_geometry: Any,
          volume_geometry: VolumeGeometry, iters: int = 100,
          verbose: bool = True, mask: Any = None,
          min_constraint: float = None, max_constraint: float = None,
          return_gpu: bool = False, callback: callable = None,
-         residual_every: int = 10):
+         residual_every: int = 10, graph: bool = False):
     """Simultaneous Iterative Reconstruction Technique (``algo.py:226-368``).
 
     ``x <- clip(mask * (x - C * A^T (R * (A x - y))))`` from ``x = 0`` with
     ``C = 1/A^T 1`` and ``R = 1/A 1``.  ``callback(i, x, residual)`` gets
     device tensors.  The residual norm is only computed (a device-to-host
     sync) every ``residual_every`` iterations when ``verbose``.
+    ``graph=True`` (not in the reference) records one iteration in a HIP
+    graph and replays it, one launch per iteration instead of four.
     """
     if len(projections) != len(projection_geometry):
         raise ValueError("Number of projections does not match number of "
                          "geometries.")
+    if mask is not None:
+        _check_mask_shape(mask, tuple(int(s) for s in volume_geometry.shape))
     state = _SirtState(projections, projection_geometry, volume_geometry,
                        mask, min_constraint, max_constraint)
+    if graph and iters > 0:
+        state.capture()
     bar = range(iters)
     if verbose:
         from tqdm import tqdm
         bar = tqdm(bar, desc="SIRT")
     res_norm = ""
     for i in bar:
-        state.iterate()
+        state.step()
         if verbose and residual_every is not None:
             if i % residual_every == 0:
                 res_norm = str(float(state.res.float().norm()))
