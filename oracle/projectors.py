@@ -74,7 +74,8 @@ def _ray_params(fr: dict, k: int, axis: int, shape):
     return a1, a2, xb0, xc0
 
 
-def cone_fp(vol: np.ndarray, pgs, vg, slice_chunk: int = 64) -> np.ndarray:
+def cone_fp(vol: np.ndarray, pgs, vg, slice_chunk: int = 64,
+            axes=None) -> np.ndarray:
     """Joseph forward projection of ``vol[x, y, z]``; returns
     ``proj[angle, row, col]`` (float64).
 
@@ -83,13 +84,18 @@ def cone_fp(vol: np.ndarray, pgs, vg, slice_chunk: int = 64) -> np.ndarray:
     (bilinear in the other two axes) and the sum is multiplied by
     ``outScale * sqrt(1 + a1^2 scale1 + a2^2 scale2)``
     (``cuda/cone_fp.cu:181-204``).
+
+    ``axes`` (optional) overrides the per-angle ray axes.  The reference
+    takes them from the detector centre (``kernels/cone_fp.py:264-285``), so
+    a part of a detector (a window cut out to check a restriction) must be
+    given the full detector's axes to sample the same way.
     """
     vol = np.asarray(vol, dtype=np.float64)
     shape = vol.shape
     if tuple(shape) != tuple(int(n) for n in vg.shape):
         raise ValueError("volume shape does not match the volume geometry")
     fr = normalized_frames(pgs, vg)
-    axes = fp_major_axes(fr)
+    axes = fp_major_axes(fr) if axes is None else np.asarray(axes)
     scales = fp_output_scale(vg.voxel_size)
     n = len(axes)
     out = np.zeros((n, fr["rows"], fr["cols"]))
@@ -111,17 +117,18 @@ def cone_fp(vol: np.ndarray, pgs, vg, slice_chunk: int = 64) -> np.ndarray:
     return out
 
 
-def cone_fp_transpose(proj: np.ndarray, pgs, vg) -> np.ndarray:
+def cone_fp_transpose(proj: np.ndarray, pgs, vg, axes=None) -> np.ndarray:
     """Exact transpose of :func:`cone_fp`: ``vol[x, y, z]`` (float64).
 
     Every (ray, slice) sample of the forward operator contributes
     ``weight * tap_weight * proj[angle, row, col]`` to each of its in-bounds
     bilinear taps, with ``weight = outScale * sqrt(1 + a1^2 s1 + a2^2 s2)``.
+    ``axes`` overrides the per-angle ray axes as in :func:`cone_fp`.
     """
     proj = np.asarray(proj, dtype=np.float64)
     shape = tuple(int(n) for n in vg.shape)
     fr = normalized_frames(pgs, vg)
-    axes = fp_major_axes(fr)
+    axes = fp_major_axes(fr) if axes is None else np.asarray(axes)
     scales = fp_output_scale(vg.voxel_size)
     out = np.zeros(shape)
     for k in range(len(axes)):

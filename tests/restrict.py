@@ -15,7 +15,12 @@ checked against the oracle on a part of it:
   coordinates).
 
 A detector window keeps the world position of every pixel it holds, so the
-per-pixel arithmetic (rays, FDK pre-weight) is unchanged.
+per-pixel arithmetic (rays, FDK pre-weight) is unchanged.  Two things a
+window does move are carried over from the full detector: the ray axis of
+each angle (chosen from the detector centre, ``kernels/cone_fp.py:264-285``:
+a window near a corner can flip it on angles near 45 degrees), passed to
+the oracle's forward projection and transpose as ``axes``, and the FDK
+pre-weight's source to detector-centre distance (:func:`fdk`).
 """
 from __future__ import annotations
 
@@ -23,6 +28,13 @@ import numpy as np
 
 import kernelkit as kk
 from kernelkit.geom.proj import Detector, ProjectionGeometry
+
+
+def ray_axes(pgs, vg):
+    """Per-angle ray axes of the full detectors (for the oracle's ``axes``
+    on windows cut from them)."""
+    import oracle
+    return oracle.fp_major_axes(oracle.normalized_frames(pgs, vg))
 
 
 def sub_detector(g, r0: int, c0: int, nr: int, nc: int):

@@ -153,7 +153,7 @@ def test_full_fp_row_bands(full):
     for r0 in (0, 380, 760):
         _say(f"full: fp rows {r0}..{r0 + 8}, angles {idx}")
         ref = oracle.cone_fp(vol, [restrict.row_band(g, r0, 8) for g in sub],
-                             vg)
+                             vg, axes=restrict.ray_axes(sub, vg))
         assert_parity(_host(y[:, r0:r0 + 8]), ref, label=f"full fp {r0}")
 
 
@@ -279,7 +279,8 @@ def test_batch64_equals_per_problem_and_oracle(batch64):
         rows = int(pg[0].detector.rows)
         r0 = rows // 2 - 4
         ref = oracle.cone_fp(vol, [restrict.row_band(pg[i], r0, 8)
-                                   for i in idx], vg)
+                                   for i in idx], vg,
+                             axes=restrict.ray_axes([pg[i] for i in idx], vg))
         assert_parity(_host(ys[k][idx, r0:r0 + 8]), ref,
                       label=f"batch fp problem {k}")
         lo = tuple(int(s) // 2 - 12 for s in vg.shape[:2]) + (
@@ -336,7 +337,8 @@ def test_large_transpose_blocks(device):
         nr, nc, org = restrict.window(sub, vg, lo, n, grow=1.5, margin=1)
         geoms, part = restrict.windowed(sub, yh, nr, nc, org)
         ref = oracle.cone_fp_transpose(part, geoms,
-                                       restrict.sub_volume(vg, lo, n))
+                                       restrict.sub_volume(vg, lo, n),
+                                       axes=restrict.ray_axes(sub, vg))
         blk = tuple(slice(a, a + k) for a, k in zip(lo, n))
         got = _host(t[blk])
         assert float(np.abs(ref).max()) > 0

@@ -49,7 +49,9 @@ def test_large_fp_bands(large, device):
     del x
     for r0 in (764, 40, 1490):
         band = [_band(g, r0, 4) for g in pgs]
-        ref = oracle.cone_fp(vol, band, vg)
+        ref = oracle.cone_fp(vol, band, vg,
+                             axes=oracle.fp_major_axes(
+                                 oracle.normalized_frames(pgs, vg)))
         assert_parity(y[:, r0:r0 + 4], ref, label=f"large fp rows {r0}")
 
 
@@ -88,5 +90,7 @@ def test_large_fp_noise_bands(large, device):
     del x
     for r0 in (766, 3):
         band = [_band(g_, r0, 2) for g_ in pgs]
-        assert_parity(y[:, r0:r0 + 2], oracle.cone_fp(vol, band, vg),
+        axes = oracle.fp_major_axes(oracle.normalized_frames(pgs, vg))
+        assert_parity(y[:, r0:r0 + 2], oracle.cone_fp(vol, band, vg,
+                                                      axes=axes),
                       label=f"large fp noise rows {r0}")

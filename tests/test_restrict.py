@@ -65,9 +65,46 @@ def test_block_transpose(problem, lo, n):
     full = oracle.cone_fp_transpose(proj, pgs, vg)
     nr, nc, org = restrict.window(pgs, vg, lo, n, grow=1.5, margin=1)
     geoms, part = restrict.windowed(pgs, proj, nr, nc, org)
-    got = oracle.cone_fp_transpose(part, geoms, sub)
+    got = oracle.cone_fp_transpose(part, geoms, sub,
+                                   axes=restrict.ray_axes(pgs, vg))
     np.testing.assert_allclose(got, full[blk], rtol=1e-12,
                                atol=1e-12 * np.abs(full).max())
+
+
+def test_window_keeps_the_full_detectors_ray_axes():
+    """Near 45 degrees a window at a detector corner has its own centre ray
+    on the other side of the diagonal: left to itself the oracle would pick
+    the other ray axis there (a different sampling).  With the full
+    detector's axes the windowed forward projection and transpose equal the
+    full ones; without them they differ."""
+    from kernelkit import workloads
+    pgs = workloads.circular_geometry(8, sod=60.0, odd=60.0, rows=20,
+                                      cols=60, pixel=1.0)
+    pgs = [pgs[1], pgs[3]]                        # 45 and 135 degrees
+    pgs = [kk.rotate(g, yaw=np.deg2rad(-3.0)) for g in pgs]
+    vg = kk.resolve_volume_geometry(shape=(24, 24, 8), voxel_size=1.0)
+    rng = np.random.default_rng(1)
+    vol = rng.uniform(size=vg.shape)
+    proj = rng.uniform(size=(2, 20, 60))
+    full_fp = oracle.cone_fp(vol, pgs, vg)
+    full_t = oracle.cone_fp_transpose(proj, pgs, vg)
+    lo, n = (0, 0, 3), (4, 4, 2)
+    nr, nc, org = restrict.window(pgs, vg, lo, n, grow=1.5, margin=1)
+    geoms, part = restrict.windowed(pgs, proj, nr, nc, org)
+    axes = restrict.ray_axes(pgs, vg)
+    assert (restrict.ray_axes(geoms, vg) != axes).any()   # the trap is set
+    blk = tuple(slice(a, a + k) for a, k in zip(lo, n))
+    sub = restrict.sub_volume(vg, lo, n)
+    got = oracle.cone_fp_transpose(part, geoms, sub, axes=axes)
+    np.testing.assert_allclose(got, full_t[blk], rtol=1e-12,
+                               atol=1e-12 * np.abs(full_t).max())
+    bad = oracle.cone_fp_transpose(part, geoms, sub)
+    assert np.abs(bad - full_t[blk]).max() > 1e-3 * np.abs(full_t).max()
+    fp = oracle.cone_fp(vol, geoms, vg, axes=axes)
+    ref = np.stack([full_fp[k, r0:r0 + nr, c0:c0 + nc]
+                    for k, (r0, c0) in enumerate(org)])
+    np.testing.assert_allclose(fp, ref, rtol=1e-12,
+                               atol=1e-12 * np.abs(full_fp).max())
 
 
 def test_window_too_small_is_detected(problem):
