@@ -245,7 +245,7 @@ __device__ __forceinline__ void trace(const kk::FpAngle& P,
 // block id `bid`.  ALL_COLC: every angle of the launch is column-coherent,
 // so a thread keeps one b slope/intercept for its R rays (fewer registers,
 // more wavefronts in flight); otherwise the angle's flag picks the path.
-template <int R, bool ALL_COLC, bool PIPE, int COLS>
+template <int R, bool ALL_COLC, bool PIPE, int COLS, bool PTRS = false>
 __device__ __forceinline__ void fp_tile_at(
     const kk::FpAngle* __restrict__ tab, int32_t ti, const FpArgs& A,
     int32_t ang, int32_t ct, int32_t rt) {
@@ -347,7 +347,7 @@ __device__ __forceinline__ void fp_tile_at(
     float val = acc[k] * wscale[k];
     const int64_t o = ang * A.os0 + row * A.os1 + col * A.os2;
     float* out = A.out;
-    if (A.out_ptrs != nullptr)
+    if constexpr (PTRS)
       out = reinterpret_cast<float*>(
           A.out_ptrs[A.ptr_axis == 0 ? ang : (A.ptr_axis == 1 ? row : col)]);
     if (A.mode == 1) {
@@ -370,7 +370,8 @@ __device__ __forceinline__ void fp_tile_at(
 // constant-memory (scalar) loads first.  A table passed as a kernel
 // argument is read in place (the compiler scalarizes it by itself; the
 // explicit copy measured 5% slower at 512^3 / 720 angles).
-template <int R, bool ALL_COLC, bool PIPE, int COLS, bool TABLE_IN_MEMORY>
+template <int R, bool ALL_COLC, bool PIPE, int COLS, bool TABLE_IN_MEMORY,
+          bool PTRS = false>
 __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
                                         const FpArgs& A, int64_t bid) {
   const int32_t ang = static_cast<int32_t>(bid % A.n_angles);
@@ -379,9 +380,9 @@ __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
   const int32_t rt = static_cast<int32_t>(t / A.col_tiles);
   if constexpr (TABLE_IN_MEMORY) {
     const kk::FpAngle rec = kk_uniform_load(tab + ang);
-    fp_tile_at<R, ALL_COLC, PIPE, COLS>(&rec, 0, A, ang, ct, rt);
+    fp_tile_at<R, ALL_COLC, PIPE, COLS, PTRS>(&rec, 0, A, ang, ct, rt);
   } else {
-    fp_tile_at<R, ALL_COLC, PIPE, COLS>(tab, ang, A, ang, ct, rt);
+    fp_tile_at<R, ALL_COLC, PIPE, COLS, PTRS>(tab, ang, A, ang, ct, rt);
   }
 }
 
@@ -399,10 +400,12 @@ constexpr int fp_batch_min_blocks(int r) {
 constexpr bool fp_pipe(int r) { return r <= 2; }
 constexpr bool fp_batch_pipe(int r) { return r <= 1; }
 
-template <int R, bool ALL_COLC>
+// PTRS: output through the table of entry pointers (kk_cone_fp_ptrs); a
+// separate instantiation, so the plain kernel's code is unchanged.
+template <int R, bool ALL_COLC, bool PTRS>
 __global__ __launch_bounds__(kColsSingle * kWave, fp_min_blocks(R)) void cone_fp_kernel(
     const kk::FpAngle* __restrict__ tab, FpArgs A) {
-  fp_tile<R, ALL_COLC, fp_pipe(R), kColsSingle, false>(
+  fp_tile<R, ALL_COLC, fp_pipe(R), kColsSingle, false, PTRS>(
       tab, A, kk_xcd_remap(blockIdx.x, A.total_blocks, A.xcd_group));
 }
 
@@ -499,22 +502,22 @@ __global__ __launch_bounds__(kColsBatch * kWave, fp_batch_min_blocks(R)) void co
   }
 }
 
-template <bool C>
+template <bool C, bool P>
 void launch_single(int32_t rpt, dim3 grid, hipStream_t stream,
                    const kk::FpAngle* tab, const FpArgs& A) {
   const dim3 block(kColsSingle * kWave);
   switch (rpt) {
     case 1:
-      hipLaunchKernelGGL((cone_fp_kernel<1, C>), grid, block, 0, stream, tab,
-                         A);
+      hipLaunchKernelGGL((cone_fp_kernel<1, C, P>), grid, block, 0, stream,
+                         tab, A);
       break;
     case 2:
-      hipLaunchKernelGGL((cone_fp_kernel<2, C>), grid, block, 0, stream, tab,
-                         A);
+      hipLaunchKernelGGL((cone_fp_kernel<2, C, P>), grid, block, 0, stream,
+                         tab, A);
       break;
     default:
-      hipLaunchKernelGGL((cone_fp_kernel<4, C>), grid, block, 0, stream, tab,
-                         A);
+      hipLaunchKernelGGL((cone_fp_kernel<4, C, P>), grid, block, 0, stream,
+                         tab, A);
       break;
   }
 }
@@ -680,10 +683,15 @@ int launch_fp_spec(const kk::FpSpec& S, int32_t rows_per_thread,
   const int rc = make_fp_args(S, rpt, kColsSingle, A);
   if (rc != KK_OK) return rc;
   const dim3 grid(static_cast<unsigned>(A.total_blocks));
-  if (S.plan->fp_all_colc)
-    launch_single<true>(rpt, grid, stream, S.plan->fp, A);
+  const bool c = S.plan->fp_all_colc, p = S.ptrs != nullptr;
+  if (c && !p)
+    launch_single<true, false>(rpt, grid, stream, S.plan->fp, A);
+  else if (!c && !p)
+    launch_single<false, false>(rpt, grid, stream, S.plan->fp, A);
+  else if (c)
+    launch_single<true, true>(rpt, grid, stream, S.plan->fp, A);
   else
-    launch_single<false>(rpt, grid, stream, S.plan->fp, A);
+    launch_single<false, true>(rpt, grid, stream, S.plan->fp, A);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_kernel");
   return KK_OK;
