@@ -66,10 +66,16 @@ struct FptArgs {
 };
 
 constexpr int kTile = 64;  // ray_weight_kernel: 64 x 64 pixels per block
+// Zero rows before and after every column of the workspace: a lane's row
+// window of up to kRowPad rows may start above or end below the detector
+// and still reads in-bounds zeros, so the column loop needs no per-row
+// bounds test.
+constexpr int kRowPad = 16;
 
-// yw[angle][col][row] = proj(angle, row, col) * kk_fp_weight of the ray,
-// 0 where kk_fp_ray rejects the ray.  Reads along columns and writes
-// along rows, both coalesced, through an LDS tile.
+// yw[angle][col][kRowPad + row] = proj(angle, row, col) * kk_fp_weight of
+// the ray, 0 where kk_fp_ray rejects the ray, and kRowPad zeros before and
+// after each column (column stride rows + 2 kRowPad).  Reads along columns
+// and writes along rows, both coalesced, through an LDS tile.
 __global__ __launch_bounds__(256) void ray_weight_kernel(
     const kk::FpAngle* __restrict__ tab, const float* __restrict__ proj,
     int64_t ps0, int64_t ps1, int64_t ps2, int32_t rows, int32_t cols,
@@ -96,11 +102,16 @@ __global__ __launch_bounds__(256) void ray_weight_kernel(
     tile[t][lane] = v;
   }
   __syncthreads();
-  float* out = yw + static_cast<int64_t>(ang) * rows * cols;
+  const int64_t cs = rows + 2 * kRowPad;
+  float* out = yw + static_cast<int64_t>(ang) * cols * cs + kRowPad;
   for (int32_t t = w; t < kTile; t += 4) {
     const int32_t col = c0 + t, r = r0 + lane;
-    if (r < rows && col < cols)
-      out[static_cast<int64_t>(col) * rows + r] = tile[lane][t];
+    if (col >= cols) continue;
+    if (r < rows) out[col * cs + r] = tile[lane][t];
+    // the pads: above the first row band, below the last
+    if (trow == 0 && lane < kRowPad) out[col * cs - kRowPad + lane] = 0.f;
+    if (trow == tiles_r - 1 && lane < kRowPad)
+      out[col * cs + rows + lane] = 0.f;
   }
 }
 
@@ -142,14 +153,6 @@ __device__ __forceinline__ void int_range(float lo, float hi, int32_t n,
   b = min(n - 1, static_cast<int32_t>(floorf(hi)));
 }
 
-// Weight of voxel `v` (unpadded index) in a sample whose taps are the
-// padded element `idx` (voxels idx - 1 and idx, weights 1 - f and f).
-__device__ __forceinline__ float tap_weight(uint32_t idx, float f,
-                                            int32_t v) {
-  const int32_t t = static_cast<int32_t>(idx) - kk::kHaloLo;
-  return t == v ? 1.f - f : (t + 1 == v ? f : 0.f);
-}
-
 // Value of lane `l` of a wavefront (l wave-uniform), in scalar registers.
 __device__ __forceinline__ int32_t bcast(int32_t v, int32_t l) {
   return __builtin_amdgcn_readlane(v, l);
@@ -167,24 +170,6 @@ __device__ __forceinline__ double bcast(double v, int32_t l) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
-// Sample of row r in a column-coherent column (D0c, inv: the column's
-// kk_fp_ray terms): adds y times its c-tap weight to the lane's kJC
-// column sums (voxels k .. k + kJC - 1).
-__device__ __forceinline__ void add_c_taps(const kk::FpAngle& P, double D0c,
-                                           double inv, double fi, double nc1,
-                                           int32_t r, int32_t k, float y,
-                                           float (&colsum)[kJC]) {
-  const double Dc = fma(static_cast<double>(r), P.vc, D0c);
-  const double a2 = Dc * inv;
-  const double b2 = fma(a2, P.sig, P.tauc);
-  uint32_t ic;
-  float fcw;
-  kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
-#pragma unroll
-  for (int cc = 0; cc < kJC; ++cc)
-    colsum[cc] = fmaf(y, tap_weight(ic, fcw, k + cc), colsum[cc]);
-}
-
 __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
     const kk::FpAngle* __restrict__ tab, FptArgs A) {
   int64_t bid = blockIdx.x;
@@ -200,10 +185,10 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
 
   const double fi = static_cast<double>(i);
   const double nb1 = static_cast<double>(A.nb + 1);
-  const double nc1 = static_cast<double>(A.nc + 1);
   const double kd = static_cast<double>(k);
   const float kf = static_cast<float>(k);
-  const int64_t angle_elems = static_cast<int64_t>(A.rows) * A.cols;
+  const int32_t cstride = A.rows + 2 * kRowPad;  // workspace column stride
+  const int64_t angle_elems = static_cast<int64_t>(cstride) * A.cols;
   // b-tap sums of this lane's voxels at t = -1 .. kJB (row t + 1; rows 0
   // and kJB + 1 catch taps just outside the b-run and are discarded),
   // indexed per lane: every lane owns its words, so no atomics and no
@@ -219,7 +204,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
   for (int32_t q = 0; q < A.n_angles; ++q) {
     const int32_t ang = A.angles[q];
     const kk::FpAngle& P = tab[ang];
-    const float* img = A.yw + ang * angle_elems;
+    const float* img = A.yw + ang * angle_elems + kRowPad;
     const double arel = fi + P.sig;  // slice centre, relative to the source
     if (P.colc) {
       // columns whose b coordinate at this slice lies in
@@ -237,16 +222,18 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       }
       for (int32_t cb = c_lo; cb <= c_hi; cb += 64) {
         const int32_t ncol = min(64, c_hi - cb + 1);
-        // ray data of column cb + lane (exactly kk_fp_ray's: the row terms
-        // of a and b are exact zeros for these angles)
-        double inv = 0.0, D0c = 0.0;
-        float fb = 0.f, ia = 0.f, beta = 0.f;
-        int32_t t0 = -99;
+        // ray data of column cb + lane (kk_fp_ray's terms: the row terms
+        // of a and b are exact zeros for these angles).  The c coordinate
+        // of the column's rays at this slice is affine in the row:
+        // x_c(r) = beta + alpha r (padded coordinates, float64).
+        double alpha = 0.0, beta = 0.0;
+        float fb = 0.f, ia = 0.f, betaf = 0.f, alphaf = 0.f, off = 0.f;
+        int32_t t0 = -99, nrow = 0;
         if (lane < ncol) {
           const double fc = static_cast<double>(cb + lane);
           const double Da = fma(fc, P.ua, P.qa);
           const double Db = fma(fc, P.ub, P.qb);
-          inv = 1.0 / Da;
+          const double inv = 1.0 / Da;
           const double a1 = Db * inv;
           const double b1 = fma(a1, P.sig, P.taub);
           if (isfinite(a1) && isfinite(b1)) {
@@ -254,46 +241,124 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
             kk_fp_coord(a1, b1, fi, nb1, ib, fb);
             t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
           }
-          D0c = fma(fc, P.uc, P.qc);
-          // c coordinate of the column's rays: alpha * row + beta
-          const float alpha = static_cast<float>(P.vc * inv * arel);
-          ia = fabsf(alpha) > 1e-9f ? 1.f / alpha : 0.f;
-          beta = static_cast<float>(fma(D0c * inv, arel, P.tauc));
+          const double D0c = fma(fc, P.uc, P.qc);
+          alpha = P.vc * inv * arel;
+          beta = fma(D0c * inv, arel, P.tauc);
+          if (!isfinite(alpha) || !isfinite(beta)) t0 = -99;
+          alphaf = static_cast<float>(alpha);
+          ia = fabsf(alphaf) > 1e-9f ? 1.f / alphaf : 0.f;
+          betaf = static_cast<float>(beta);
+          // a lane's window: the rows whose c coordinate lies in the
+          // padded span [k, k + kJC + 1] of its voxels' tents, (kJC + 1)
+          // |1 / alpha| rows long.  Windows of at most kRowPad rows are
+          // read as nrow rows from floor(first row) (the extra rows get
+          // zero tent weight); longer ones take the exact range below.
+          const float len = static_cast<float>(kJC + 1) * fabsf(ia);
+          if (ia != 0.f && len + 3.f <= static_cast<float>(kRowPad)) {
+            nrow = static_cast<int32_t>(floorf(len + 2e-3f)) + 2;
+            off = fminf(0.f, static_cast<float>(kJC + 1) * ia) - 1e-3f;
+          }
         }
         for (int32_t m = 0; m < ncol; ++m) {
           const int32_t tm = bcast(t0, m);
           if (tm < -1 || tm >= kJB) continue;
-          const double invm = bcast(inv, m), D0cm = bcast(D0c, m);
           const float iam = bcast(ia, m);
-          // rows whose c coordinate lies in [k, k + kJC + 1), found in
-          // float32 and widened by 1e-2 row (float32 error ~1e-4 row); the
-          // samples themselves are evaluated exactly below
+          const int32_t nr = bcast(nrow, m);
+          const float* ycol = img + (cb + m) * cstride;
+          float colsum[kJC];
+#pragma unroll
+          for (int cc = 0; cc < kJC; ++cc) colsum[cc] = 0.f;
+          if (nr > 0) {
+            // nr rows (wave-uniform: no per-lane loop control) from the
+            // floor of the window's first row, clamped into the padded
+            // column (a window off the detector reads zeros)
+            const float r1 =
+                fmaf(kf - bcast(betaf, m), iam, bcast(off, m));
+            const int32_t r_lo =
+                min(max(static_cast<int32_t>(floorf(r1)), -kRowPad),
+                    A.rows + kRowPad - nr);
+            // the window's first row in float64, relative to the centre
+            // of the lane's first voxel (padded coordinate k + 1); the
+            // rows after it in float32 offsets (|alpha u| is a few voxels,
+            // so the offsets carry ~1e-7 voxel of rounding).  A voxel's
+            // weight is the tent 1 - |x_c - centre| of the sample, equal
+            // to the forward kernel's bilinear tap weight.
+            const float alm = bcast(alphaf, m);
+            const float loc = static_cast<float>(
+                fma(bcast(alpha, m), static_cast<double>(r_lo),
+                    bcast(beta, m)) - (kd + 1.0));
+            const float* yw = ycol + r_lo;
+            int32_t u = 0;
+            for (; u + kRU <= nr; u += kRU) {
+              float y[kRU];
+#pragma unroll
+              for (int t = 0; t < kRU; ++t) y[t] = yw[u + t];
+#pragma unroll
+              for (int t = 0; t < kRU; ++t) {
+                const float x = fmaf(alm, static_cast<float>(u + t), loc);
+#pragma unroll
+                for (int cc = 0; cc < kJC; ++cc)
+                  colsum[cc] = fmaf(
+                      y[t],
+                      fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
+                      colsum[cc]);
+              }
+            }
+            for (; u < nr; ++u) {
+              const float yv = yw[u];
+              const float x = fmaf(alm, static_cast<float>(u), loc);
+#pragma unroll
+              for (int cc = 0; cc < kJC; ++cc)
+                colsum[cc] = fmaf(
+                    yv, fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
+                    colsum[cc]);
+            }
+          } else {
+          // rows whose c coordinate lies in [k, k + kJC + 1) (the padded
+          // span of the lane's voxels' tents), found in float32 and
+          // widened by 1e-2 row; samples outside get zero weight below
           int32_t r_lo = 0, r_hi = A.rows - 1;
           if (iam != 0.f) {
-            const float r1 = (kf - bcast(beta, m)) * iam;
+            const float r1 = (kf - bcast(betaf, m)) * iam;
             const float r2 = fmaf(static_cast<float>(kJC + 1), iam, r1);
             int_range(fminf(r1, r2) - 1e-2f, fmaxf(r1, r2) + 1e-2f, A.rows,
                       r_lo, r_hi);
           }
-          const float* ycol = img + (cb + m) * A.rows;
-          float colsum[kJC];
-#pragma unroll
-          for (int cc = 0; cc < kJC; ++cc) colsum[cc] = 0.f;
-          // rows in groups of kRU, loads first, then one at a time (a
-          // lane's rows are contiguous, typically kRU + 1 of them), summed
-          // in row order; rays the forward kernel skips have yw = 0 (and
-          // clamped, finite coordinates here)
+          // the lane's first row in float64 (the first voxel's centre is
+          // at padded coordinate k + 1), float32 offsets after it, tents
+          // as above; rays the forward kernel skips have yw = 0
+          const float alm = bcast(alphaf, m);
+          const float loc = static_cast<float>(
+              fma(bcast(alpha, m), static_cast<double>(r_lo),
+                  bcast(beta, m)) - (kd + 1.0));
           int32_t r = r_lo;
+          float du = 0.f;
           for (; r + (kRU - 1) <= r_hi; r += kRU) {
             float y[kRU];
 #pragma unroll
             for (int u = 0; u < kRU; ++u) y[u] = ycol[r + u];
 #pragma unroll
-            for (int u = 0; u < kRU; ++u)
-              add_c_taps(P, D0cm, invm, fi, nc1, r + u, k, y[u], colsum);
+            for (int u = 0; u < kRU; ++u) {
+              const float x = fmaf(alm, du + static_cast<float>(u), loc);
+#pragma unroll
+              for (int cc = 0; cc < kJC; ++cc)
+                colsum[cc] = fmaf(
+                    y[u], fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
+                    colsum[cc]);
+            }
+            du += static_cast<float>(kRU);
           }
-          for (; r <= r_hi; ++r)
-            add_c_taps(P, D0cm, invm, fi, nc1, r, k, ycol[r], colsum);
+          for (; r <= r_hi; ++r) {
+            const float yv = ycol[r];
+            const float x = fmaf(alm, du, loc);
+#pragma unroll
+            for (int cc = 0; cc < kJC; ++cc)
+              colsum[cc] = fmaf(
+                  yv, fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
+                  colsum[cc]);
+            du += 1.f;
+          }
+          }
           // the column's two b taps (tm wave-uniform): read-modify-write
           // of the lane's own LDS words
           const float fbm = bcast(fb, m);
@@ -316,7 +381,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
                           P.taub;
         const double xc = (corner & 2 ? kd + (kJC + 1) + kEps : kd - kEps) -
                           P.tauc;
-        double pc, pr;
+        double pc = 0.0, pr = 0.0;
         ok = ok && project(P, arel, xb, xc, pc, pr);
         cmin = fmin(cmin, pc);
         cmax = fmax(cmax, pc);
@@ -328,52 +393,78 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         int_range(cmin, cmax, A.cols, c_lo, c_hi);
         int_range(rmin, rmax, A.rows, r_lo, r_hi);
       }
-      for (int32_t col = c_lo; col <= c_hi; ++col) {
-        const float* ycol = img + col * A.rows;
-        // the column terms of kk_fp_ray; its row terms are added per row in
-        // the same nested order, so every value is bit-identical to it
-        const double fc = static_cast<double>(col);
-        const double Da0 = fma(fc, P.ua, P.qa);
-        const double Db0 = fma(fc, P.ub, P.qb);
-        const double Dc0 = fma(fc, P.uc, P.qc);
+      // columns in a wave-uniform loop (the union of the lanes' ranges),
+      // rows per lane.  Sample coordinates: float64 at an anchor pixel
+      // (the wave's first column, the lane's first row), float32 offsets
+      // from it.  With D = q + col U + row V (kk_fp_ray's terms) and
+      // t = D_b / D_a at the anchor, x_b = arel D_b / D_a + tau_b moves by
+      // arel (dD_b - t dD_a) / D_a between the anchor and a pixel: a
+      // quantity of the box's size, so its float32 rounding is ~1e-6
+      // voxel.  Weights are the tents 1 - |x - centre| of the forward
+      // kernel's bilinear taps.
+      const int32_t wc_lo = kk_wave_min(c_lo <= c_hi ? c_lo : A.cols);
+      const int32_t wc_hi = kk_wave_max(c_lo <= c_hi ? c_hi : -1);
+      const double fwc = static_cast<double>(wc_lo);
+      const double fr0 = static_cast<double>(r_lo);
+      const double Da0 = fma(fr0, P.va, fma(fwc, P.ua, P.qa));
+      const double Db0 = fma(fr0, P.vb, fma(fwc, P.ub, P.qb));
+      const double Dc0 = fma(fr0, P.vc, fma(fwc, P.uc, P.qc));
+      const double inv0 = 1.0 / Da0;
+      const double tb = Db0 * inv0, tcv = Dc0 * inv0;
+      const float ob =
+          static_cast<float>(fma(tb, arel, P.taub) - (jw + kk::kHaloLo));
+      const float oc =
+          static_cast<float>(fma(tcv, arel, P.tauc) - (kd + kk::kHaloLo));
+      const float da0 = static_cast<float>(Da0);
+      const float tbf = static_cast<float>(tb), tcf = static_cast<float>(tcv);
+      const float arf = static_cast<float>(arel);
+      const float uaf = static_cast<float>(P.ua), ubf = static_cast<float>(P.ub),
+                  ucf = static_cast<float>(P.uc);
+      const float vaf = static_cast<float>(P.va), vbf = static_cast<float>(P.vb),
+                  vcf = static_cast<float>(P.vc);
+      const bool anchor_ok = ok && isfinite(tb) && isfinite(tcv);
+      for (int32_t col = wc_lo; col <= wc_hi; ++col) {
+        const float* ycol = img + col * cstride;
+        const float dcf = static_cast<float>(col - wc_lo);
+        const float dAc = dcf * uaf, dBc = dcf * ubf, dCc = dcf * ucf;
+        const int32_t rl = r_lo;
+        const int32_t rh = (anchor_ok && col >= c_lo && col <= c_hi) ? r_hi
+                                                                    : r_lo - 1;
         // b-tap sums over a run of samples with the same tap index, added
         // to the lane's LDS words when the index changes (along a column
         // it rarely does)
-        int32_t tcur = -1;
+        int32_t tcur = -2;  // no run yet
         float s0[kJC], s1[kJC];
 #pragma unroll
         for (int cc = 0; cc < kJC; ++cc) s0[cc] = s1[cc] = 0.f;
-        for (int32_t r = r_lo; r <= r_hi; ++r) {
-          const double fr = static_cast<double>(r);
-          const double inv = 1.0 / fma(fr, P.va, Da0);
-          const double a1 = fma(fr, P.vb, Db0) * inv;
-          const double a2 = fma(fr, P.vc, Dc0) * inv;
-          const double b1 = fma(a1, P.sig, P.taub);
-          const double b2 = fma(a2, P.sig, P.tauc);
-          // rays kk_fp_ray rejects (non-finite terms) have yw = 0, and
-          // their clamped coordinates keep every index in range
-          uint32_t ic;
-          float fcw;
-          kk_fp_coord(a2, b2, fi, nc1, ic, fcw);
+        float drf = 0.f;
+        for (int32_t r = rl; r <= rh; ++r, drf += 1.f) {
+          const float dA = fmaf(drf, vaf, dAc);
+          const float dB = fmaf(drf, vbf, dBc);
+          const float dC = fmaf(drf, vcf, dCc);
+          const float sc = arf * __builtin_amdgcn_rcpf(da0 + dA);
+          const float xb = fmaf(fmaf(-tbf, dA, dB), sc, ob);
+          const float xc = fmaf(fmaf(-tcf, dA, dC), sc, oc);
           float wk[kJC];
           bool any = false;
 #pragma unroll
           for (int cc = 0; cc < kJC; ++cc) {
-            wk[cc] = tap_weight(ic, fcw, k + cc);
-            any = any || wk[cc] != 0.f;
+            wk[cc] = fmaxf(0.f, 1.f - fabsf(xc - static_cast<float>(cc)));
+            any = any || wk[cc] > 0.f;
           }
           if (!any) continue;
-          uint32_t ib;
-          float fb;
-          kk_fp_coord(a1, b1, fi, nb1, ib, fb);
-          const int32_t t0 = static_cast<int32_t>(ib) - kk::kHaloLo - jw;
-          if (t0 < -1 || t0 >= kJB) continue;
+          const float fl = floorf(xb);
+          if (!(fl >= -1.f && fl < static_cast<float>(kJB))) continue;
+          const int32_t t0 = static_cast<int32_t>(fl);
+          const float fb = xb - fl;
           if (t0 != tcur) {
+            if (tcur >= -1) {
 #pragma unroll
-            for (int cc = 0; cc < kJC; ++cc) {
-              g[tcur + 1][cc][lane] += s0[cc];
-              g[tcur + 2][cc][lane] += s1[cc];
-              s0[cc] = s1[cc] = 0.f;
+              for (int cc = 0; cc < kJC; ++cc) {
+                g[tcur + 1][cc][lane] += s0[cc];
+                g[tcur + 2][cc][lane] += s1[cc];
+                s0[cc] = s1[cc] = 0.f;
+              }
             }
             tcur = t0;
           }
@@ -385,10 +476,12 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
             s1[cc] = fmaf(fb, val, s1[cc]);
           }
         }
+        if (tcur >= -1) {
 #pragma unroll
-        for (int cc = 0; cc < kJC; ++cc) {
-          g[tcur + 1][cc][lane] += s0[cc];
-          g[tcur + 2][cc][lane] += s1[cc];
+          for (int cc = 0; cc < kJC; ++cc) {
+            g[tcur + 1][cc][lane] += s0[cc];
+            g[tcur + 2][cc][lane] += s1[cc];
+          }
         }
       }
     }
@@ -417,7 +510,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
 int64_t kk::fp_transpose_work_floats(const kk_plan_s* plan, int32_t rows,
                                      int32_t cols) {
   if (plan == nullptr || rows <= 0 || cols <= 0) return -1;
-  return int64_t(plan->fp_n) * rows * cols;
+  return int64_t(plan->fp_n) * cols * (rows + 2 * kRowPad);
 }
 
 int kk::fp_transpose_launch(const kk_plan_s* plan, const float* proj,
@@ -495,4 +588,10 @@ extern "C" int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
   return kk::fp_transpose_launch(plan, proj, ps, rows, cols, vol, vol_shape,
                                  vs, additive, work,
                                  static_cast<hipStream_t>(stream));
+}
+
+extern "C" int64_t kk_cone_fp_transpose_work_elems(const kk_plan* plan,
+                                                   int32_t rows,
+                                                   int32_t cols) {
+  return kk::fp_transpose_work_floats(plan, rows, cols);
 }

@@ -2,9 +2,9 @@
 // call hipMalloc (code under HIP graph capture, tight serving loops,
 // harnesses that forbid allocation).
 //
-// kk_cone_fp_transpose needs a workspace of n_angles * rows * cols floats
-// (the weighted projections, column-major per angle), which its callers
-// allocate.  Here that workspace is a file-scope device array of fixed size,
+// kk_cone_fp_transpose needs a workspace of about n_angles * rows * cols
+// floats (the weighted projections, column-major per angle), which its
+// callers allocate.  Here that workspace is a file-scope device array of fixed size,
 // `scratch`.  Each device the library runs on holds its own copy of it, at
 // its own address, until the process ends.  In host code the name `scratch`
 // is a host-side stand-in, not a device address, so it is never passed to a
@@ -13,8 +13,8 @@
 //
 // Every call on one device shares that device's array and leaves the last
 // call's contents in it.  The transpose overwrites every workspace element
-// it reads (the ray-weight pass writes all n_angles * rows * cols values
-// before the gather reads them, in stream order), so nothing needs clearing
+// it reads (the ray-weight pass writes every workspace value, pads
+// included, before the gather reads them, in stream order), so nothing needs clearing
 // between calls.  Calls that overlap on one device (other streams, other
 // host threads) race on the array.
 #include <mutex>
@@ -74,7 +74,7 @@ extern "C" int kk_cone_fp_transpose_static(const kk_plan* plan,
                     "kk_cone_fp_transpose_static: bad detector");
   if (need > kScratchFloats)
     return kk::fail(KK_ERR_UNSUPPORTED,
-                    "kk_cone_fp_transpose_static: %lld angles x %d x %d "
+                    "kk_cone_fp_transpose_static: %lld angles of %d x %d "
                     "detector pixels need a workspace of %lld floats, more "
                     "than the static scratch of %lld floats; use "
                     "kk_cone_fp_transpose with a caller workspace",

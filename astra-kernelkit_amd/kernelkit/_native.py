@@ -101,6 +101,8 @@ _SIGNATURES = {
         _vp, _vp, _c_i64p, ctypes.c_int32, ctypes.c_int32, _vp, _c_i32p,
         _c_i64p, ctypes.c_int32, _vp]),
     "kk_cone_fp_transpose_static_capacity": (ctypes.c_int64, []),
+    "kk_cone_fp_transpose_work_elems": (ctypes.c_int64, [
+        _vp, ctypes.c_int32, ctypes.c_int32]),
     "kk_cone_bp":(ctypes.c_int, [_vp, _vp, ctypes.c_int32, ctypes.c_int32,
                                   ctypes.c_int32, _vp, _c_i32p, _c_i64p,
                                   ctypes.c_float, ctypes.c_int32, _vp]),

@@ -435,7 +435,11 @@ class RayDrivenConeFP(BaseKernel):
                 _native.stream_ptr(stream)))
             return
         # the library's scratch: weighted projections, column-major
-        work = torch.empty(n * rows * cols, dtype=torch.float32,
+        n_work = int(_native.lib().kk_cone_fp_transpose_work_elems(
+            self._plan.ptr, rows, cols))
+        if n_work < 0:
+            raise ValueError(f"bad detector size {rows} x {cols}")
+        work = torch.empty(n_work, dtype=torch.float32,
                            device=projections.device)
         if stream is not None:
             work.record_stream(stream)

@@ -382,7 +382,18 @@ def autograd_diff():
     t_ag = _event_ms(fwd_bwd(fpm), stream, reps=3)
     t_agu = _event_ms(fwd_bwd(fpu), stream, reps=3)
     t_t = _event_ms(lambda: A.matched_T(y, out=v), stream, reps=3)
+    t_fp = _event_ms(lambda: A(x, out=y), stream, reps=3)
     updates = int(np.prod(vg.shape)) * len(pgs)
+    # the general-geometry path of the transpose: the same sizes on an
+    # orbit with seeded +-1 degree wobble and +-2 pixel detector drift
+    c = dict(workloads.CONFIGS["diff"])
+    c.pop("shape"), c.pop("n_angles")
+    pgw = workloads.circular_geometry(len(pgs), wobble_deg=1.0, drift_px=2.0,
+                                      seed=1, **c)
+    Aw = kk.XrayTransform(pgw, vg)
+    yw = Aw(x)
+    t_fpw = _event_ms(lambda: Aw(x, out=yw), stream, reps=3)
+    t_tw = _event_ms(lambda: Aw.matched_T(yw, out=v), stream, reps=3)
     return {
         "workload": "256^3 volume, 360 angles, 384^2 detector",
         "plain_fp_plus_bp_ms": round(t_plain, 3),
@@ -392,6 +403,11 @@ def autograd_diff():
         "unmatched_ratio_vs_plain": round(t_agu / t_plain, 3),
         "transpose_ms": round(t_t, 3),
         "transpose_gups": round(updates / (t_t * 1e-3) / 1e9, 1),
+        "fp_ms": round(t_fp, 3),
+        "transpose_vs_fp": round(t_t / t_fp, 3),
+        "wobble_fp_ms": round(t_fpw, 3),
+        "wobble_transpose_ms": round(t_tw, 3),
+        "wobble_transpose_vs_fp": round(t_tw / t_fpw, 3),
     }
 
 

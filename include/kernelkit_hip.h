@@ -153,13 +153,18 @@ int kk_cone_bp(const kk_plan* plan, const float* proj_tex, int32_t n_angles,
  * where an exact gradient is wanted.  proj: (angle, row, col) strides; vol:
  * world strides, shape equal to the parameters' volume shape.  additive !=
  * 0: vol += A^T proj, else vol = A^T proj.  work: caller-owned device
- * scratch of n_angles * rows * cols floats (overwritten; the weighted
- * projections in column-major order).  rows * cols < 2^31. */
+ * scratch of kk_cone_fp_transpose_work_elems(plan, rows, cols) floats
+ * (overwritten; the weighted projections in column-major order with zero
+ * rows around each column).  rows * cols < 2^31. */
 int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
                          const int64_t proj_strides[3], int32_t rows,
                          int32_t cols, float* vol, const int32_t vol_shape[3],
                          const int64_t vol_strides[3], int32_t additive,
                          float* work, void* stream);
+/* Floats of kk_cone_fp_transpose's workspace for the plan's angles and a
+ * rows x cols detector (host-only; -1 on bad arguments). */
+int64_t kk_cone_fp_transpose_work_elems(const kk_plan* plan, int32_t rows,
+                                        int32_t cols);
 
 /* kk_cone_fp_transpose without the caller workspace: it allocates nothing
  * (no hipMalloc, no library allocation).  No counterpart in the reference;
@@ -168,7 +173,7 @@ int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
  * The workspace is a static device array of
  * kk_cone_fp_transpose_static_capacity() floats (64 Mi floats, 256 MiB) that
  * every device the library runs on holds until the process ends; problems
- * with n_angles * rows * cols above it are rejected with
+ * whose kk_cone_fp_transpose_work_elems is above it are rejected with
  * KK_ERR_UNSUPPORTED.  All calls on one device share that array: calls that
  * overlap on one device (other streams or host threads) race.  Arguments,
  * results and errors are otherwise those of kk_cone_fp_transpose. */

@@ -365,6 +365,29 @@ def _fpt_taps(P, i, rows, cols, nb, nc):
     return [np.concatenate(x) for x in zip(*out)]
 
 
+def _circ_window(ia, betaf, kl, rows, pad=16):
+    """Circular path: the rows cone_fp_t_kernel reads for a lane (first c
+    voxel kl) and a column (float32 1 / alpha and beta): nrow rows from the
+    floor of the window's first row when the window is short, else the
+    exact float32 range widened by 1e-2 row."""
+    ia = np.asarray(ia, f32)
+    with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
+        ln = (f32(_KJC + 1) * np.abs(ia)).astype(f32)
+        fast = (ia != 0) & (ln + f32(3) <= f32(pad))
+        nrow = np.floor(ln + f32(2e-3)).astype(np.int64) + 2
+        off = (np.minimum(f32(0), f32(_KJC + 1) * ia) - f32(1e-3)).astype(f32)
+        r1f = _f32fma((kl.astype(f32) - betaf).astype(f32), ia, off)
+        rf = np.floor(np.where(fast, r1f, 0)).astype(np.int64)
+        rf = np.minimum(np.maximum(rf, -pad), rows + pad - nrow)
+        r1 = ((kl.astype(f32) - betaf) * ia).astype(f32)
+        r2 = _f32fma(_KJC + 1.0, ia, r1)
+    r_lo, r_hi = _int_range(np.minimum(r1, r2) - f32(1e-2),
+                            np.maximum(r1, r2) + f32(1e-2), rows)
+    steep = ia != 0
+    r_lo, r_hi = np.where(steep, r_lo, 0), np.where(steep, r_hi, rows - 1)
+    return (np.where(fast, rf, r_lo), np.where(fast, rf + nrow - 1, r_hi))
+
+
 def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
     """Whether cone_fp_t_kernel's lane owning voxel (i, J, K) visits ray
     (R, C): the candidate enumeration of both of its paths, with the
@@ -393,12 +416,7 @@ def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
         steep = np.abs(alpha) > f32(1e-9)
         with np.errstate(divide="ignore", over="ignore"):
             ia = np.where(steep, f32(1) / alpha, f32(0))
-            r1 = (K.astype(f32) - beta) * ia
-            # fmaf(kJC + 1, ia, r1): exact in float64, rounded once
-            r2 = ((_KJC + 1.0) * ia.astype(np.float64) + r1).astype(f32)
-        r_lo, r_hi = _int_range(np.minimum(r1, r2) - f32(1e-2),
-                                np.maximum(r1, r2) + f32(1e-2), rows)
-        r_lo, r_hi = np.where(steep, r_lo, 0), np.where(steep, r_hi, rows - 1)
+        r_lo, r_hi = _circ_window(ia, beta, K, rows)
     else:
         cs, rs, oks = [], [], []
         for corner in range(4):
@@ -416,15 +434,112 @@ def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
             (c_hi - c_lo + 1) * (r_hi - r_lo + 1))
 
 
-def emulate_fp_t(proj, table, shape, eps=_KEPS):
+def _f32fma(a, b, c):
+    """fmaf: a * b + c rounded once to float32 (the float64 product of two
+    float32 values is exact)."""
+    return (np.asarray(a, np.float64) * np.asarray(b, np.float64)
+            + np.asarray(c, np.float64)).astype(f32)
+
+
+def _lane_boxes(P, i, jw, kl, rows, cols, eps=_KEPS):
+    """General path: each lane's detector box (c_lo, c_hi, r_lo, r_hi) and
+    whether its corners projected."""
+    arel = float(i) + float(P["sig"])
+    cs, rs, oks = [], [], []
+    for corner in range(4):
+        xb = (jw + _KJB + 1 + eps if corner & 1 else jw - eps) - \
+            float(P["taub"])
+        xc = (kl + _KJC + 1.0 + eps if corner & 2 else kl - eps) - \
+            float(P["tauc"])
+        pc, pr, ok = _project_corner(P, arel, xb, xc)
+        cs.append(pc), rs.append(pr), oks.append(ok)
+    ok = np.logical_and.reduce(oks)
+    with np.errstate(invalid="ignore"):
+        c_lo, c_hi = _int_range(np.min(cs, 0), np.max(cs, 0), cols)
+        r_lo, r_hi = _int_range(np.min(rs, 0), np.max(rs, 0), rows)
+    c_lo, c_hi = np.where(ok, c_lo, 0), np.where(ok, c_hi, cols - 1)
+    r_lo, r_hi = np.where(ok, r_lo, 0), np.where(ok, r_hi, rows - 1)
+    return c_lo, c_hi, r_lo, r_hi, ok
+
+
+def _fpt_kernel_weights(P, i, R, C, J, K, rows, cols, nb):
+    """The tap weights cone_fp_t_kernel forms for taps (R, C) -> (J, K):
+    float64 anchors, float32 offsets and tents 1 - |x - centre| (without
+    the ray weight), as its two paths compute them."""
+    lo = _native.HALO_LO
+    jw = (J // _KJB) * _KJB
+    kl = (K // _KJC) * _KJC
+    arel = float(i) + float(P["sig"])
+    g = {x: float(P[x]) for x in ("qa", "qb", "qc", "ua", "ub", "uc", "va",
+                                  "vb", "vc", "sig", "taub", "tauc")}
+    if int(P["colc"]):
+        fc = C.astype(np.float64)
+        Da = fc * g["ua"] + g["qa"]
+        Db = fc * g["ub"] + g["qb"]
+        inv = 1.0 / Da
+        a1 = Db * inv
+        ib, fb = _coord(a1, a1 * g["sig"] + g["taub"], i, nb + 1)
+        t0 = ib - lo - jw
+        D0c = fc * g["uc"] + g["qc"]
+        alpha = g["vc"] * inv * arel
+        beta = D0c * inv * arel + g["tauc"]
+        alphaf, betaf = alpha.astype(f32), beta.astype(f32)
+        with np.errstate(divide="ignore", over="ignore"):
+            ia = np.where(np.abs(alphaf) > f32(1e-9), f32(1) / alphaf, f32(0))
+        r_lo, _ = _circ_window(ia, betaf, kl, rows)
+        loc = (alpha * r_lo + beta - (kl + 1.0)).astype(f32)
+        x = _f32fma(alphaf, (R - r_lo).astype(f32), loc)
+        wc = np.maximum(f32(0), f32(1) - np.abs((x - (K - kl)).astype(f32)))
+        fb64 = fb
+    else:
+        c_lo, _, r_lo, _, _ = _lane_boxes(P, i, jw, kl, rows, cols)
+        # the wave's first column: the least c_lo over its 64 lanes
+        wave = kl // (64 * _KJC)
+        wc_lo = np.empty_like(C)
+        for jw_u, w_u in set(zip(jw.tolist(), wave.tolist())):
+            lanes = w_u * 64 * _KJC + _KJC * np.arange(64)
+            cl, ch, _, _, _ = _lane_boxes(P, i, np.full(64, jw_u), lanes,
+                                          rows, cols)
+            m = (jw == jw_u) & (wave == w_u)
+            wc_lo[m] = np.where(cl <= ch, cl, cols).min()
+        Da0 = r_lo * g["va"] + (wc_lo * g["ua"] + g["qa"])
+        Db0 = r_lo * g["vb"] + (wc_lo * g["ub"] + g["qb"])
+        Dc0 = r_lo * g["vc"] + (wc_lo * g["uc"] + g["qc"])
+        tb, tc = Db0 / Da0, Dc0 / Da0
+        ob = (tb * arel + g["taub"] - (jw + lo)).astype(f32)
+        oc = (tc * arel + g["tauc"] - (kl + lo)).astype(f32)
+        F = {x: f32(g[x]) for x in ("ua", "ub", "uc", "va", "vb", "vc")}
+        dcf = (C - wc_lo).astype(f32)
+        drf = (R - r_lo).astype(f32)
+        dA = _f32fma(drf, F["va"], (dcf * F["ua"]).astype(f32))
+        dB = _f32fma(drf, F["vb"], (dcf * F["ub"]).astype(f32))
+        dC = _f32fma(drf, F["vc"], (dcf * F["uc"]).astype(f32))
+        rcp = (1.0 / (Da0.astype(f32) + dA).astype(np.float64)).astype(f32)
+        sc = (f32(arel) * rcp).astype(f32)
+        xb = _f32fma(_f32fma(-tb.astype(f32), dA, dB), sc, ob)
+        xc = _f32fma(_f32fma(-tc.astype(f32), dA, dC), sc, oc)
+        wc = np.maximum(f32(0), f32(1) - np.abs((xc - (K - kl)).astype(f32)))
+        fl = np.floor(xb)
+        t0 = fl.astype(np.int64)
+        fb64 = (xb - fl).astype(f32)
+    wb = np.where(J - jw == t0, f32(1) - fb64,
+                  np.where(J - jw == t0 + 1, fb64, f32(0))).astype(f32)
+    return (wb * wc).astype(f32)
+
+
+def emulate_fp_t(proj, table, shape, eps=_KEPS, kernel_weights=False):
     """cone_fp_t_kernel: each voxel sums the taps of the samples its lane
     visits (float64 sums; the kernel's are float32 in its own order).
-    Returns (A^T proj, taps not visited, mean candidates per lane-angle /
-    detector pixels) per path."""
+    With ``kernel_weights`` the tap weights are the kernel's own float32
+    ones (:func:`_fpt_kernel_weights`), else the forward kernel's exact
+    taps.  Returns (A^T proj, taps not visited, mean candidates per
+    lane-angle / detector pixels per path, largest |kernel weight - exact
+    weight|)."""
     n, rows, cols = proj.shape
     out = np.zeros(shape)
     missed = 0
     frac = {0: [], 1: []}
+    werr = 0.0
     for a, (_, b, c) in _FRAMES.items():
         na, nb, nc = shape[a], shape[b], shape[c]
         acc = np.zeros((na, nb, nc))
@@ -434,11 +549,25 @@ def emulate_fp_t(proj, table, shape, eps=_KEPS):
                 R, C, J, K, w = _fpt_taps(P, i, rows, cols, nb, nc)
                 hit, cand = _fpt_covered(P, i, R, C, J, K, rows, cols, eps)
                 missed += int((~hit).sum())
-                np.add.at(acc[i], (J[hit], K[hit]),
-                          proj[k, R[hit], C[hit]].astype(f32) * w[hit])
+                if kernel_weights:
+                    R, C, J, K, w = (x[hit] for x in (R, C, J, K, w))
+                    wk = _fpt_kernel_weights(P, i, R, C, J, K, rows, cols,
+                                             nb)
+                    a1, _, a2, _ = _fp_rays(P, R.astype(np.float64),
+                                            C.astype(np.float64))
+                    wr = _fp_weight(P, a1, a2).astype(np.float64)
+                    if len(w):
+                        werr = max(werr, float(np.abs(wk * wr - w).max()
+                                               / wr.max()))
+                    np.add.at(acc[i], (J, K),
+                              proj[k, R, C].astype(f32) * wk * wr)
+                else:
+                    np.add.at(acc[i], (J[hit], K[hit]),
+                              proj[k, R[hit], C[hit]].astype(f32) * w[hit])
                 frac[int(P["colc"])].append(cand.mean() / (rows * cols))
         out += np.transpose(acc, np.argsort((a, b, c)))
-    return out, missed, {p: float(np.mean(v)) for p, v in frac.items() if v}
+    return (out, missed, {p: float(np.mean(v)) for p, v in frac.items() if v},
+            werr)
 
 
 @pytest.mark.parametrize("pert", [{}, {"tilt_deg": 2.0, "drift_px": 4.0}])
@@ -453,11 +582,17 @@ def test_fp_transpose_emulation(pert):
     assert set(table["axis"].tolist()) >= {2}
     assert bool(table["colc"][:5].all()) == (not pert)
     y = np.random.default_rng(3).uniform(size=(len(pgs), 30, 44))
-    got, missed, frac = emulate_fp_t(y, table, vg.shape)
+    got, missed, frac, _ = emulate_fp_t(y, table, vg.shape)
     assert missed == 0
     assert all(f < 0.25 for f in frac.values()), frac
-    assert_parity(got, oracle.cone_fp_transpose(y, pgs, vg),
-                  label=f"fp_t-emu {pert}")
+    ref = oracle.cone_fp_transpose(y, pgs, vg)
+    assert_parity(got, ref, label=f"fp_t-emu {pert}")
+    # the kernel's float32 offsets from float64 anchors and its tent
+    # weights: every tap weight within 5e-6 (of the largest ray weight) of
+    # the forward kernel's float64-coordinate taps
+    got, _, _, werr = emulate_fp_t(y, table, vg.shape, kernel_weights=True)
+    assert werr < 5e-6, werr
+    assert_parity(got, ref, label=f"fp_t-emu kernel weights {pert}")
     # windows a quarter voxel short on each side lose taps: the coverage
     # check can fail
     assert emulate_fp_t(y, table, vg.shape, eps=-0.25)[1] > 0
@@ -670,3 +805,44 @@ def test_uniform_i2d_matches_int_to_double():
     got = _uniform_i2d_bits(vals.astype(np.uint64))
     want = vals.astype(np.float64).view(np.uint64)
     np.testing.assert_array_equal(got, want)
+
+
+@pytest.mark.parametrize("name,wobble,angles,slices", [
+    ("diff", 0.0, (0, 77), (3, 128, 250)),
+    ("diff", 1.0, (5, 200), (0, 131, 255)),
+    ("large", None, (9,), (517,))])
+def test_fp_transpose_kernel_weights_at_scale(name, wobble, angles, slices):
+    """The transpose kernel's float32 tap weights at the differentiation
+    (256^3, 384^2, source-origin 1000) and large (1024^3, 1536^2, wobbling)
+    workloads: within 5e-6 of the forward kernel's on every tap of the
+    sampled slices (a sample of the taps for the large one)."""
+    from kernelkit.workloads import CONFIGS, circular_geometry
+    c = dict(CONFIGS[name])
+    shape = c.pop("shape")
+    c.pop("n_angles")
+    if wobble is not None:
+        c.update(wobble_deg=wobble, drift_px=2.0 if wobble else 0.0)
+    n = 1440 if name == "large" else 360
+    pgs = circular_geometry(n, seed=1, **c)
+    vg = kk.resolve_volume_geometry(shape=shape, voxel_size=1.0)
+    pgs = [pgs[k] for k in angles]
+    table = _fp_table(pgs, vg)
+    rows, cols = int(c["rows"]), int(c["cols"])
+    rng = np.random.default_rng(0)
+    for P in table:
+        a = int(P["axis"])
+        _, b, cc = _FRAMES[a]
+        nb, nc = shape[b], shape[cc]
+        for i in slices:
+            R, C, J, K, w = _fpt_taps(P, i, rows, cols, nb, nc)
+            if len(R) > 300000:
+                pick = rng.choice(len(R), 300000, replace=False)
+                R, C, J, K, w = (x[pick] for x in (R, C, J, K, w))
+            hit, _ = _fpt_covered(P, i, R, C, J, K, rows, cols)
+            assert hit.all()
+            wk = _fpt_kernel_weights(P, i, R, C, J, K, rows, cols, nb)
+            a1, _, a2, _ = _fp_rays(P, R.astype(np.float64),
+                                    C.astype(np.float64))
+            wr = _fp_weight(P, a1, a2).astype(np.float64)
+            err = np.abs(wk * wr - w) / wr
+            assert err.max() < 5e-6, (name, int(P["colc"]), i, err.max())

@@ -40,7 +40,9 @@ def _both(device, op, seed):
     st = _native.stream_ptr()
     pst = _native.i64(_logical_strides(y, (0, 1, 2)))
     vst = _native.i64(out_ref.stride())
-    work = torch.empty(n * rows * cols, dtype=torch.float32, device=device)
+    n_work = int(lib.kk_cone_fp_transpose_work_elems(K._plan.ptr, rows, cols))
+    assert n_work >= n * rows * cols
+    work = torch.empty(n_work, dtype=torch.float32, device=device)
     _native.check(lib.kk_cone_fp_transpose(
         K._plan.ptr, _native._vp(y.data_ptr()), pst, rows, cols,
         _native._vp(out_ref.data_ptr()), _native.i32(world), vst, 0,
