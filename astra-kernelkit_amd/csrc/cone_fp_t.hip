@@ -248,34 +248,39 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
           alphaf = static_cast<float>(alpha);
           ia = fabsf(alphaf) > 1e-9f ? 1.f / alphaf : 0.f;
           betaf = static_cast<float>(beta);
-          // a lane's window: the rows whose c coordinate lies in the
-          // padded span [k, k + kJC + 1] of its voxels' tents, (kJC + 1)
-          // |1 / alpha| rows long.  Windows of at most kRowPad rows are
-          // read as nrow rows from floor(first row) (the extra rows get
+          // a lane's window: the rows whose c coordinate lies in the open
+          // padded span (k, k + kJC + 1) of its voxels' tents, (kJC + 1)
+          // |1 / alpha| rows long, i.e. at most floor(len) + 1 integer
+          // rows.  Windows of at most kRowPad - 3 rows are read as nrow
+          // rows from the ceiling of the first row, moved 1e-3 row earlier
+          // (float32 rounding of the start is ~1e-5 row; rows outside get
           // zero tent weight); longer ones take the exact range below.
           const float len = static_cast<float>(kJC + 1) * fabsf(ia);
           if (ia != 0.f && len + 3.f <= static_cast<float>(kRowPad)) {
-            nrow = static_cast<int32_t>(floorf(len + 2e-3f)) + 2;
+            nrow = static_cast<int32_t>(floorf(len + 2e-3f)) + 1;
             off = fminf(0.f, static_cast<float>(kJC + 1) * ia) - 1e-3f;
           }
         }
+        // the column's tap row and window length, one broadcast
+        const int32_t tpk = (min(max(t0, -99), 100) + 128) | (nrow << 8);
         for (int32_t m = 0; m < ncol; ++m) {
-          const int32_t tm = bcast(t0, m);
+          const int32_t pk = bcast(tpk, m);
+          const int32_t tm = (pk & 255) - 128;
           if (tm < -1 || tm >= kJB) continue;
           const float iam = bcast(ia, m);
-          const int32_t nr = bcast(nrow, m);
+          const int32_t nr = pk >> 8;
           const float* ycol = img + (cb + m) * cstride;
           float colsum[kJC];
 #pragma unroll
           for (int cc = 0; cc < kJC; ++cc) colsum[cc] = 0.f;
           if (nr > 0) {
             // nr rows (wave-uniform: no per-lane loop control) from the
-            // floor of the window's first row, clamped into the padded
+            // ceiling of the window's first row, clamped into the padded
             // column (a window off the detector reads zeros)
             const float r1 =
                 fmaf(kf - bcast(betaf, m), iam, bcast(off, m));
             const int32_t r_lo =
-                min(max(static_cast<int32_t>(floorf(r1)), -kRowPad),
+                min(max(static_cast<int32_t>(ceilf(r1)), -kRowPad),
                     A.rows + kRowPad - nr);
             // the window's first row in float64, relative to the centre
             // of the lane's first voxel (padded coordinate k + 1); the
@@ -289,13 +294,15 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
                     bcast(beta, m)) - (kd + 1.0));
             const float* yw = ycol + r_lo;
             int32_t u = 0;
-            for (; u + kRU <= nr; u += kRU) {
+            float uf = 0.f;   // u as a float, counted (exact)
+            for (; u + kRU <= nr; u += kRU, uf += static_cast<float>(kRU)) {
               float y[kRU];
 #pragma unroll
               for (int t = 0; t < kRU; ++t) y[t] = yw[u + t];
 #pragma unroll
               for (int t = 0; t < kRU; ++t) {
-                const float x = fmaf(alm, static_cast<float>(u + t), loc);
+                const float x =
+                    fmaf(alm, uf + static_cast<float>(t), loc);
 #pragma unroll
                 for (int cc = 0; cc < kJC; ++cc)
                   colsum[cc] = fmaf(
@@ -304,9 +311,9 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
                       colsum[cc]);
               }
             }
-            for (; u < nr; ++u) {
+            for (; u < nr; ++u, uf += 1.f) {
               const float yv = yw[u];
-              const float x = fmaf(alm, static_cast<float>(u), loc);
+              const float x = fmaf(alm, uf, loc);
 #pragma unroll
               for (int cc = 0; cc < kJC; ++cc)
                 colsum[cc] = fmaf(

@@ -368,16 +368,16 @@ def _fpt_taps(P, i, rows, cols, nb, nc):
 def _circ_window(ia, betaf, kl, rows, pad=16):
     """Circular path: the rows cone_fp_t_kernel reads for a lane (first c
     voxel kl) and a column (float32 1 / alpha and beta): nrow rows from the
-    floor of the window's first row when the window is short, else the
+    ceiling of the window's first row when the window is short, else the
     exact float32 range widened by 1e-2 row."""
     ia = np.asarray(ia, f32)
     with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
         ln = (f32(_KJC + 1) * np.abs(ia)).astype(f32)
         fast = (ia != 0) & (ln + f32(3) <= f32(pad))
-        nrow = np.floor(ln + f32(2e-3)).astype(np.int64) + 2
+        nrow = np.floor(ln + f32(2e-3)).astype(np.int64) + 1
         off = (np.minimum(f32(0), f32(_KJC + 1) * ia) - f32(1e-3)).astype(f32)
         r1f = _f32fma((kl.astype(f32) - betaf).astype(f32), ia, off)
-        rf = np.floor(np.where(fast, r1f, 0)).astype(np.int64)
+        rf = np.ceil(np.where(fast, r1f, 0)).astype(np.int64)
         rf = np.minimum(np.maximum(rf, -pad), rows + pad - nrow)
         r1 = ((kl.astype(f32) - betaf) * ia).astype(f32)
         r2 = _f32fma(_KJC + 1.0, ia, r1)
