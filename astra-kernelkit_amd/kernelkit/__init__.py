@@ -3,8 +3,10 @@
 Drop-in for the public namespace of the reference's ``kernelkit/__init__.py``
 (:1-18): operators, projectors, algorithms (``fp``, ``bp``, ``fdk``,
 ``sirt``), geometry, data helpers and the ``kernel``/``kernels``/
-``processing`` modules.  Device arrays are PyTorch tensors on the HIP device;
-compute runs in hand-written HIP kernels reached through a ctypes C ABI.
+``processing`` modules.  Device arrays are PyTorch tensors on the HIP device,
+any ``__cuda_array_interface__`` array, or :class:`DeviceArray` (the
+package's own, allocated by the library: PyTorch is optional); compute runs
+in hand-written HIP kernels reached through a ctypes C ABI.
 """
 __version__ = "1.0.0-alpha.2+mi355x"
 
@@ -20,6 +22,8 @@ from .projector import (BackProjector, BaseProjector, ForwardProjector,  # noqa:
                         VolumeNotSetError)
 from .algo import fp, bp, fbp, fdk, sirt  # noqa: E402,F401
 from .data import *  # noqa: E402,F401,F403
+from .device import DeviceArray  # noqa: E402,F401
+from . import device  # noqa: E402,F401
 from . import kernel  # noqa: E402,F401
 from . import kernels  # noqa: E402,F401
 from . import experimental  # noqa: E402,F401

@@ -5,14 +5,17 @@ runtime (``kernelkit/kernel.py:188-226``).  The library is built ahead of
 time by ``build_native.py``; if it is missing, every device operation raises
 :class:`NativeLibraryError` -- there is no CPU fallback.
 
-Device memory and streams come from PyTorch: arrays are ``torch.Tensor`` on
-the HIP device, streams are ``torch.cuda`` streams (their raw ``hipStream_t``
-is passed through).
+Every entry point takes raw device pointers, element strides and a raw
+``hipStream_t``; the arrays may be torch tensors, ``__cuda_array_interface__``
+arrays or :class:`kernelkit.DeviceArray` (memory from the library's own
+``kk_malloc``), and streams torch streams, objects with a ``cuda_stream`` /
+``ptr`` attribute, or plain integers.
 """
 from __future__ import annotations
 
 import ctypes
 import os
+import sys
 from pathlib import Path
 
 import numpy as np
@@ -25,6 +28,9 @@ KK_ERR_HIP = -2
 KK_ERR_NOT_SET = -3
 KK_ERR_SHAPE = -4
 KK_ERR_UNSUPPORTED = -5
+KK_COPY_H2D = 1
+KK_COPY_D2H = 2
+KK_COPY_D2D = 3
 HALO_LO = 1
 HALO_HI = 2
 HALO = HALO_LO + HALO_HI
@@ -66,6 +72,16 @@ _SIGNATURES = {
                                                     ctypes.c_int32, _c_i64p,
                                                     _vp, _vp]),
     "kk_fill": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_float, _vp]),
+    "kk_device_count": (ctypes.c_int, [_c_i32p]),
+    "kk_get_device": (ctypes.c_int, [_c_i32p]),
+    "kk_set_device": (ctypes.c_int, [ctypes.c_int32]),
+    "kk_malloc": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int64]),
+    "kk_free": (ctypes.c_int, [_vp]),
+    "kk_memcpy": (ctypes.c_int, [_vp, _vp, ctypes.c_int64, ctypes.c_int32,
+                                 _vp]),
+    "kk_stream_synchronize": (ctypes.c_int, [_vp]),
+    "kk_copy_strided": (ctypes.c_int, [_vp, _c_i64p, _vp, _c_i64p, _c_i64p,
+                                       ctypes.c_int32, _vp]),
     "kk_cone_fp": (ctypes.c_int, [_vp, _vp, _vp, _c_i64p, ctypes.c_int32,
                                   ctypes.c_int32, ctypes.c_int32, _vp]),
     "kk_cone_fp_ptrs": (ctypes.c_int, [_vp, _vp, _vp, ctypes.c_int32,
@@ -94,6 +110,9 @@ _SIGNATURES = {
     "kk_batch_bp": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "kk_probe_l2_read": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32,
                                         ctypes.c_int32, _vp, _vp]),
+    "kk_probe_gather": (ctypes.c_int, [
+        _vp, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32,
+        ctypes.c_int32, ctypes.c_int32, _vp, _vp]),
     "kk_cone_fp_transpose": (ctypes.c_int, [
         _vp, _vp, _c_i64p, ctypes.c_int32, ctypes.c_int32, _vp, _c_i32p,
         _c_i64p, ctypes.c_int32, _vp, _vp]),
@@ -233,11 +252,23 @@ def i32_ptr(arr: np.ndarray):
 
 
 def stream_ptr(stream=None) -> ctypes.c_void_p:
-    """Raw hipStream_t of a torch stream (default: the current stream)."""
-    import torch
+    """Raw hipStream_t of ``stream``: a torch stream, an object carrying
+    the handle as ``cuda_stream`` or ``ptr`` (CuPy's), or an integer.  By
+    default torch's current stream when PyTorch is loaded and has set up
+    its HIP runtime, else the null stream (torch is never imported here)."""
     if stream is None:
-        stream = torch.cuda.current_stream()
-    return ctypes.c_void_p(int(stream.cuda_stream))
+        torch = sys.modules.get("torch")
+        if torch is not None and torch.cuda.is_initialized():
+            stream = torch.cuda.current_stream()
+        else:
+            return ctypes.c_void_p(None)
+    if isinstance(stream, int):
+        return ctypes.c_void_p(stream or None)
+    for attr in ("cuda_stream", "ptr"):
+        handle = getattr(stream, attr, None)
+        if handle is not None:
+            return ctypes.c_void_p(int(handle) or None)
+    raise TypeError(f"not a stream: {type(stream).__name__}")
 
 
 class Plan:

@@ -18,7 +18,7 @@ from typing import Any, Callable, Sequence
 
 import numpy as np
 
-from kernelkit import _native
+from kernelkit import _native, device
 from kernelkit.geom.proj import Beam, ProjectionGeometry
 from kernelkit.geom.vol import VolumeGeometry
 from kernelkit.kernel import (BaseKernel, ProjectionTexture, as_device_array,
@@ -30,7 +30,7 @@ from kernelkit.projector import BackProjector, ForwardProjector
 
 
 def _to_host(t, out=None):
-    host = t.detach().cpu().numpy()
+    host = device.to_host(t)
     if out is not None:
         out[...] = host
         return out
@@ -64,16 +64,16 @@ def bp(projections: Any, projection_geometry, volume_geometry: VolumeGeometry,
     row-filtered (fused, into the backprojector's texture).  Returns a host
     array unless ``out`` is a device tensor.
     """
-    import torch
     projection_axes = tuple(projection_axes)
-    if is_device_array(out):
-        out_dev = out
-    else:
-        out_dev = torch.empty(tuple(int(s) for s in volume_geometry.shape),
-                              dtype=torch.float32, device=default_device())
     proj = as_device_array(projections)
+    if is_device_array(out):
+        out_dev = as_device_array(out)
+    else:
+        default_device()
+        out_dev = device.empty(tuple(int(s) for s in volume_geometry.shape),
+                               like=proj)
     if preproc_fn is not None:
-        if proj is projections:
+        if is_device_array(projections):   # the caller's memory: copy first
             proj = proj.clone()
         preproc_fn(proj, projection_geometry)
     kwargs.pop("beam", None)
@@ -85,7 +85,7 @@ def bp(projections: Any, projection_geometry, volume_geometry: VolumeGeometry,
     if filter is not None:
         if projection_axes != (0, 1, 2):
             raise NotImplementedError()
-        tex = ProjectionTexture(*proj.shape, device=proj.device)
+        tex = ProjectionTexture(*proj.shape, like=proj)
         tex.zero()
         filter_into(proj, tex, geoms=list(projection_geometry),
                     filter=filter)
@@ -149,7 +149,6 @@ class _SirtState:
 
     def __init__(self, projections, projection_geometry, volume_geometry,
                  mask, min_constraint, max_constraint):
-        import torch
         if mask is not None:
             _check_mask_shape(mask, tuple(int(s) for s in
                                           volume_geometry.shape))
@@ -159,9 +158,9 @@ class _SirtState:
         rows = int(self.pg[0].detector.rows)
         cols = int(self.pg[0].detector.cols)
         self.n, self.rows, self.cols = n, rows, cols
-        dev = default_device()
+        default_device()            # raises without a HIP device
         if isinstance(projections, (list, tuple)):
-            y = torch.stack([as_device_array(p) for p in projections])
+            y = device.stack([as_device_array(p) for p in projections])
         else:
             y = as_device_array(projections)
         if tuple(y.shape) != (n, rows, cols):
@@ -176,16 +175,15 @@ class _SirtState:
         self.bptor = BackProjector(texture_type="pitch2d")
         self.bptor.projection_geometry = self.pg
         self.bptor.volume_geometry = volume_geometry
-        self.x = torch.zeros(shape, dtype=torch.float32, device=dev)
+        # every buffer is of the measurements' kind (torch or not)
+        self.x = device.zeros(shape, like=y)
         # C = 1/BP(1), R = 1/FP(1) with inf -> 0 (algo.py:320-333)
-        C = torch.zeros(shape, dtype=torch.float32, device=dev)
-        self.bptor.projections = torch.ones((n, rows, cols),
-                                            dtype=torch.float32, device=dev)
+        C = device.zeros(shape, like=y)
+        self.bptor.projections = device.full((n, rows, cols), 1.0, like=y)
         self.bptor.volume = C
         self.bptor()
-        R = torch.empty((n, rows, cols), dtype=torch.float32, device=dev)
-        self.fptor.volume = torch.ones(shape, dtype=torch.float32,
-                                       device=dev)
+        R = device.empty((n, rows, cols), like=y)
+        self.fptor.volume = device.full(shape, 1.0, like=y)
         self.fptor.projections = R
         self.fptor()
         lib = _native.lib()
@@ -202,7 +200,7 @@ class _SirtState:
         self.rtex.zero()
         self.res = self.rtex.values()
         # a broadcast view: stride 0 along the mask's broadcast axes
-        self.mask = (None if mask is None else torch.broadcast_to(
+        self.mask = (None if mask is None else device.broadcast_to(
             as_device_array(mask), shape))
         self.lo = 0.0 if min_constraint is None else float(min_constraint)
         self.hi = 0.0 if max_constraint is None else float(max_constraint)
@@ -210,8 +208,7 @@ class _SirtState:
         self.clip_hi = 0 if max_constraint is None else 1
         self.scale = float(volume_geometry.voxel_volume())
         # forward-projection buffer of the residual step
-        self.p = torch.empty((n, rows, cols), dtype=torch.float32,
-                             device=dev)
+        self.p = device.empty((n, rows, cols), like=y)
 
     def iterate(self):
         lib = _native.lib()
@@ -304,7 +301,8 @@ def sirt(projections: Any, projection_geometry: Any,
         state.step()
         if verbose and residual_every is not None:
             if i % residual_every == 0:
-                res_norm = str(float(state.res.float().norm()))
+                res_norm = str(float(np.linalg.norm(
+                    device.to_host(state.res))))
             bar.set_description(f"SIRT: {res_norm}")
         if callback is not None:
             callback(i, state.x, state.res)

@@ -1,26 +1,28 @@
 """Pitched-row helpers (drop-in for ``kernelkit/data.py`` of the reference).
 
 A pitched array pads its last axis so every row starts on a 32-byte boundary
-(``data.py:5-27``).  Works on numpy arrays and PyTorch tensors; for tensors
-the pitched buffer is the view's ``_base`` (or the tensor itself).
+(``data.py:5-27``).  Works on numpy arrays and device arrays (torch tensors,
+:class:`~kernelkit.DeviceArray`); the pitched buffer is the view's base
+(``base`` for numpy and device arrays, ``_base`` for tensors) or the array
+itself.
 """
 from __future__ import annotations
 
 import numpy as np
 
+from kernelkit import device as _dev
+
 _PITCH_BYTES = 32  # row alignment of pitched buffers, as in the reference
 
 
-def _is_tensor(x) -> bool:
-    try:
-        import torch
-    except ImportError:  # pragma: no cover
-        return False
-    return isinstance(x, torch.Tensor)
+def _on_device(x) -> bool:
+    return not isinstance(x, np.ndarray) and _dev._is_device(x)
 
 
 def _itemsize(array) -> int:
-    return array.element_size() if _is_tensor(array) else array.dtype.itemsize
+    if _on_device(array):
+        return _dev.asarray(array).element_size()
+    return np.dtype(array.dtype).itemsize
 
 
 def pitched_shape(array) -> tuple:
@@ -33,9 +35,10 @@ def pitched_shape(array) -> tuple:
 
 
 def _base(array):
-    if _is_tensor(array):
+    if _dev.is_torch(array):
         return array._base if array._base is not None else array
-    return array.base if array.base is not None else array
+    base = getattr(array, "base", None)
+    return base if base is not None else array
 
 
 def ispitched(array) -> bool:
@@ -48,22 +51,21 @@ def aspitched(array, xp=None):
     """Copy into a zero-padded pitched buffer; return the unpadded view.
 
     ``xp`` selects the output kind: ``None`` (same as the input), ``np`` /
-    ``"numpy"`` for host memory, or ``"torch"``/``"cuda"`` for device
-    memory.
+    ``"numpy"`` for host memory, or ``"cuda"`` (or ``"torch"``) for device
+    memory (of the input's kind for device inputs; a torch tensor for host
+    inputs when PyTorch is loaded, else a :class:`~kernelkit.DeviceArray`).
     """
     want_dev = _want_device(array, xp)
-    if ispitched(array) and _is_tensor(array) == want_dev:
+    if ispitched(array) and _on_device(array) == want_dev:
         return array
     if want_dev:
-        import torch
         from kernelkit.kernel import as_device_array
         src = as_device_array(array)
-        buf = torch.zeros(pitched_shape(src), dtype=src.dtype,
-                          device=src.device)
-        buf[..., :src.shape[-1]] = src
-        return buf[..., :src.shape[-1]]
-    src = array.detach().cpu().numpy() if _is_tensor(array) \
-        else np.asarray(array)
+        buf = _dev.zeros(pitched_shape(src), like=src)
+        view = buf[..., :src.shape[-1]]
+        view.copy_(src)
+        return view
+    src = _dev.to_host(array) if _on_device(array) else np.asarray(array)
     buf = np.zeros(pitched_shape(src), dtype=src.dtype)
     buf[..., :src.shape[-1]] = src
     return buf[..., :src.shape[-1]]
@@ -77,22 +79,30 @@ def pitched_like(array, xp=None, fill=None):
     want_dev = _want_device(array, xp)
     shape = pitched_shape(array)
     if want_dev:
-        import torch
         from kernelkit.kernel import default_device
-        dtype = array.dtype if _is_tensor(array) else torch.float32
-        base = torch.empty(shape, dtype=dtype, device=default_device())
+        default_device()
+        like = array if _on_device(array) else None
+        torch = _dev._torch_module()
+        if like is None and torch is not None:
+            base = torch.empty(shape, dtype=torch.float32,
+                               device=default_device())
+        else:
+            base = _dev.empty(shape, like=like)
     else:
-        dtype = array.dtype if not _is_tensor(array) else np.float32
+        dtype = array.dtype if not _on_device(array) else np.float32
         base = np.empty(shape, dtype=dtype)
     view = base[..., :array.shape[-1]]
     if fill is not None:
-        view[...] = fill
+        if _on_device(view):
+            view.fill_(fill)
+        else:
+            view[...] = fill
     return view
 
 
 def _want_device(array, xp) -> bool:
     if xp is None:
-        return _is_tensor(array)
+        return _on_device(array)
     if xp is np or xp == "numpy":
         return False
     return True

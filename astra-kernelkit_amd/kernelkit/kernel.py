@@ -21,6 +21,7 @@ from dataclasses import dataclass
 import numpy as np
 
 from kernelkit import _native
+from kernelkit import device as _dev
 
 
 class KernelNotCompiledException(Exception):
@@ -31,37 +32,82 @@ class KernelMemoryUnpreparedException(Exception):
     """Raised when a kernel is called before memory has been prepared."""
 
 
-def _torch():
-    import torch
-    return torch
+def _no_device():
+    return _native.NativeLibraryError(
+        "no HIP device available: the cone-beam operators run on the GPU "
+        "only")
 
 
 def default_device():
-    """The HIP device used for new arrays (torch's current device)."""
-    torch = _torch()
-    if not torch.cuda.is_available():
-        raise _native.NativeLibraryError(
-            "no HIP device available: the cone-beam operators run on the GPU "
-            "only")
-    return torch.device("cuda", torch.cuda.current_device())
+    """The HIP device for new arrays: torch's current device when PyTorch
+    is loaded in the process, else the library's current device."""
+    torch = _dev._torch_module()
+    if torch is not None:
+        if not torch.cuda.is_available():
+            raise _no_device()
+        return torch.device("cuda", torch.cuda.current_device())
+    if _dev.device_count() == 0:
+        raise _no_device()
+    return _dev.Device(_dev.current_device())
 
 
 def is_device_array(x) -> bool:
-    torch = _torch()
-    return isinstance(x, torch.Tensor) and x.is_cuda
+    """A torch tensor on the HIP device, a :class:`~kernelkit.DeviceArray`
+    or any array exposing ``__cuda_array_interface__``."""
+    return _dev._is_device(x)
+
+
+def _is_f32(x) -> bool:
+    return _dev.dtype_name(x) == "float32"
 
 
 def as_device_array(x, dtype=None, copy: bool = False):
-    """Device float32 tensor from a tensor or array-like (copies host data)."""
-    torch = _torch()
-    dtype = dtype or torch.float32
-    if isinstance(x, torch.Tensor):
+    """Float32 device array for ``x``.
+
+    Device arrays are used in place (``__cuda_array_interface__`` arrays
+    through a :class:`~kernelkit.DeviceArray` view; a torch tensor of
+    another dtype is converted, as torch does).  Host data is copied: into
+    a torch tensor when PyTorch is loaded in the process, else into a
+    :class:`~kernelkit.DeviceArray`.
+    """
+    if _dev.is_torch(x):
+        torch = _dev._torch_module()
+        dtype = dtype or torch.float32
         if x.is_cuda and x.dtype == dtype and not copy:
             return x
         return x.to(device=default_device() if not x.is_cuda else x.device,
                     dtype=dtype, copy=copy)
-    arr = np.asarray(x, dtype=np.float32)
-    return torch.from_numpy(np.ascontiguousarray(arr)).to(default_device())
+    if _dev._is_device(x):
+        a = _dev.asarray(x)
+        if not _is_f32(a):
+            raise ValueError(
+                f"Array must be of type float32, got {_dev.dtype_name(a)}.")
+        return a.clone() if copy else a
+    arr = np.ascontiguousarray(np.asarray(x, dtype=np.float32))
+    torch = _dev._torch_module()
+    if torch is not None and torch.cuda.is_available():
+        return torch.from_numpy(arr).to(default_device())
+    if _dev.device_count() == 0:
+        raise _no_device()
+    return _dev.from_host(arr)
+
+
+def _unsqueeze0(a):
+    if _dev.is_torch(a):
+        return a.unsqueeze(0)
+    return _dev.DeviceArray((1,) + tuple(a.shape), a.dtype,
+                            ptr=a.data_ptr(), strides=(0,) + a.stride(),
+                            owner=a, device=a.device.index)
+
+
+def _alloc(n: int, like=None, device=None):
+    """``n`` floats of device memory of the kind of ``like`` (or of the
+    torch ``device`` given)."""
+    if like is None and device is not None and not isinstance(
+            device, _dev.Device):
+        torch = _dev._torch_module()
+        return torch.empty(int(n), dtype=torch.float32, device=device)
+    return _dev.empty((int(n),), like=like)
 
 
 class Texture:
@@ -72,13 +118,12 @@ class Texture:
     """
 
     def __init__(self, array, layered: bool = False):
-        torch = _torch()
         dev = as_device_array(array)
-        if dev.dtype != torch.float32:
+        if not _is_f32(dev):
             raise ValueError(
                 f"Array must be of type float32, got {dev.dtype} instead.")
         if dev.dim() == 2:
-            dev = dev.unsqueeze(0)
+            dev = _unsqueeze0(dev)
         self.array = dev.contiguous().clone()
         self.shape = tuple(self.array.shape)
         self.layered = bool(layered)
@@ -88,7 +133,7 @@ class Texture:
         """Refresh the snapshot with new data of the same shape."""
         dev = as_device_array(array)
         if dev.dim() == 2:
-            dev = dev.unsqueeze(0)
+            dev = _unsqueeze0(dev)
         if tuple(dev.shape) != self.shape:
             raise ValueError(f"array shape {tuple(dev.shape)} does not "
                              f"match texture shape {self.shape}")
@@ -101,8 +146,7 @@ class VolumeTexture:
     (value, neighbour) float2 pairs with a zero halo per ray axis in
     ``mask`` (bit a = world axis a); see ``include/kernelkit_hip.h``."""
 
-    def __init__(self, world_shape, mask: int, device=None):
-        torch = _torch()
+    def __init__(self, world_shape, mask: int, device=None, like=None):
         self.world_shape = tuple(int(s) for s in world_shape)
         self.mask = int(mask)
         n = _native.lib().kk_fp_texture_elems(_native.i32(self.world_shape),
@@ -110,8 +154,7 @@ class VolumeTexture:
         if n < 0:
             raise ValueError(f"bad volume shape {self.world_shape} or frame "
                              f"mask {mask}")
-        self.data = torch.empty(int(n), dtype=torch.float32,
-                                device=device or default_device())
+        self.data = _alloc(n, like, device)
 
     @property
     def ptr(self):
@@ -122,10 +165,8 @@ class VolumeTexture:
 
     def update(self, volume, volume_axes=(0, 1, 2), stream=None) -> None:
         """Copy ``volume`` (array axis k = world axis ``volume_axes[k]``)."""
-        torch = _torch()
-        if not (isinstance(volume, torch.Tensor) and volume.is_cuda):
-            volume = as_device_array(volume)
-        if volume.dtype != torch.float32:
+        volume = as_device_array(volume)
+        if not _is_f32(volume):
             raise ValueError(
                 f"Volume must be of type float32, got {volume.dtype}.")
         wshape = [0, 0, 0]
@@ -147,14 +188,13 @@ class ProjectionTexture:
     (value, next-row) float2 pairs with a zero halo; see
     ``include/kernelkit_hip.h``."""
 
-    def __init__(self, n_angles: int, rows: int, cols: int, device=None):
-        torch = _torch()
+    def __init__(self, n_angles: int, rows: int, cols: int, device=None,
+                 like=None):
         self.shape = (int(n_angles), int(rows), int(cols))
         n = _native.lib().kk_bp_texture_elems(*self.shape)
         if n < 0:
             raise ValueError(f"bad projection shape {self.shape}")
-        self.data = torch.empty(int(n), dtype=torch.float32,
-                                device=device or default_device())
+        self.data = _alloc(n, like, device)
 
     @property
     def ptr(self):
@@ -169,10 +209,8 @@ class ProjectionTexture:
                stream=None) -> None:
         """Copy projections (array axis k = logical axis
         ``projection_axes[k]`` of (angle, row, col))."""
-        torch = _torch()
-        if not (isinstance(projections, torch.Tensor) and projections.is_cuda):
-            projections = as_device_array(projections)
-        if projections.dtype != torch.float32:
+        projections = as_device_array(projections)
+        if not _is_f32(projections):
             raise ValueError("Projections must be of type float32, got "
                              f"{projections.dtype}.")
         lshape = [0, 0, 0]
@@ -192,9 +230,16 @@ class ProjectionTexture:
         """The stored projections as an (angle, row, col) strided view."""
         n, rows, cols = self.shape
         h = _native.HALO
-        full = self.data.view(n, rows + h, cols + h, 2)
         lo = _native.HALO_LO
-        return full[:, lo:lo + rows, lo:lo + cols, 0]
+        if _dev.is_torch(self.data):
+            full = self.data.view(n, rows + h, cols + h, 2)
+            return full[:, lo:lo + rows, lo:lo + cols, 0]
+        st = (2 * (rows + h) * (cols + h), 2 * (cols + h), 2)
+        off = 2 * (lo * (cols + h) + lo)
+        return _dev.DeviceArray((n, rows, cols), np.float32,
+                                ptr=self.data.data_ptr() + 4 * off,
+                                strides=st, owner=self.data,
+                                device=self.data.device.index)
 
 
 def copy_to_texture(array, texture_type: str = "array",
@@ -204,14 +249,11 @@ def copy_to_texture(array, texture_type: str = "array",
     ``texture_type`` 'array' expects a 3-D array, 'array2d'/'pitch2d' a 2-D
     one (stored as a single layer); pitched or strided views are accepted.
     """
-    torch = _torch()
-    if not isinstance(array, (np.ndarray, torch.Tensor)):
+    if not (isinstance(array, np.ndarray) or is_device_array(array)):
         raise TypeError(
-            f"Expected a NumPy array or a device tensor, got {type(array)} "
+            f"Expected a NumPy array or a device array, got {type(array)} "
             "instead.")
-    dtype_ok = (array.dtype == np.float32 if isinstance(array, np.ndarray)
-                else array.dtype == torch.float32)
-    if not dtype_ok:
+    if _dev.dtype_name(array) != "float32":
         raise ValueError(
             f"Array must be of type float32, got {array.dtype} instead.")
     kind = texture_type.lower()

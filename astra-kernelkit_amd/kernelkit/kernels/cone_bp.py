@@ -16,9 +16,10 @@ from kernelkit import _native
 from kernelkit.geom import normalize_
 from kernelkit.geom.proj import ProjectionGeometrySequence
 from kernelkit.geom.vol import VolumeGeometry
+from kernelkit import device as _dev
 from kernelkit.kernel import (BaseKernel, KernelMemoryUnpreparedException,
                               KernelNotCompiledException, ProjectionTexture,
-                              Texture, is_device_array)
+                              Texture, as_device_array, is_device_array)
 
 
 def _world_strides(tensor, volume_axes) -> list:
@@ -144,14 +145,13 @@ class VoxelDrivenConeBP(BaseKernel):
         Like the reference kernel the result is accumulated
         (``cuda/cone_bp.cu:156-158``) unless ``additive=False``.
         """
-        import torch
-        if isinstance(volume, torch.Tensor) and volume.is_cuda:
-            if volume.dtype != torch.float32:
-                raise NotImplementedError(
-                    f"Volume has a dtype '{volume.dtype}'. However, currently"
-                    " there is only support for dtype=float32.")
-        else:
-            raise TypeError("`volume` must be a device tensor.")
+        if not is_device_array(volume):
+            raise TypeError("`volume` must be a device array.")
+        if _dev.dtype_name(volume) != "float32":
+            raise NotImplementedError(
+                f"Volume has a dtype '{volume.dtype}'. However, currently"
+                " there is only support for dtype=float32.")
+        volume = _dev.asarray(volume)
         if not volume_geometry.has_isotropic_voxels():
             raise NotImplementedError(
                 f"`{self.__class__.__name__}` is not tested with anisotropic "
@@ -187,12 +187,12 @@ class VoxelDrivenConeBP(BaseKernel):
                            ) -> ProjectionTexture:
         """Build (or refresh) the backprojection texture of projections
         given in ``projection_axes`` order."""
+        projections = as_device_array(projections)
         logical = [0, 0, 0]
         for k, ax in enumerate(self._proj_axs):
             logical[ax] = projections.shape[k]
         if texture is None or texture.shape != tuple(logical):
-            dev = projections.device if is_device_array(projections) else None
-            texture = ProjectionTexture(*logical, device=dev)
+            texture = ProjectionTexture(*logical, like=projections)
         texture.update(projections, self._proj_axs)
         return texture
 

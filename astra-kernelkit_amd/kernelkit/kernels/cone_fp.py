@@ -12,12 +12,14 @@ import copy
 import numpy as np
 
 from kernelkit import _native
+from kernelkit import device as _dev
 from kernelkit.geom import normalize_
 from kernelkit.geom.proj import ProjectionGeometrySequence
 from kernelkit.geom.vol import VolumeGeometry
 from kernelkit.kernel import (BaseKernel, KernelMemoryUnpreparedException,
                               KernelNotCompiledException, Texture,
-                              VolumeTexture, is_device_array)
+                              VolumeTexture, as_device_array,
+                              is_device_array)
 
 
 class FPParams(tuple):
@@ -208,10 +210,10 @@ class RayDrivenConeFP(BaseKernel):
         first upload the frames of the x and y ray axes are built."""
         mask = self.frame_mask or 0b011
         shape = tuple(int(s) for s in volume_geometry.shape)
+        volume = as_device_array(volume)
         if (texture is None or texture.world_shape != shape
                 or not texture.covers(mask)):
-            dev = volume.device if is_device_array(volume) else None
-            texture = VolumeTexture(shape, mask, device=dev)
+            texture = VolumeTexture(shape, mask, like=volume)
         texture.update(volume, self._vol_axs)
         return texture
 
@@ -275,10 +277,10 @@ class RayDrivenConeFP(BaseKernel):
                               additive, stream)
             return
         if not is_device_array(projections):
-            raise TypeError("Projection data must be a device tensor or a "
+            raise TypeError("Projection data must be a device array or a "
                             f"list of them, got {type(projections)}.")
-        import torch
-        if projections.dtype == torch.int64:
+        projections = _dev.asarray(projections)
+        if _dev.dtype_name(projections) == "int64":
             # the reference's form: a device array of entry base addresses
             # along the first projection axis, with the pitched logical
             # dimensions (kernels/cone_fp.py:173, :233-243)
@@ -295,7 +297,7 @@ class RayDrivenConeFP(BaseKernel):
             self._launch_ptrs(volume_texture, projections.contiguous(),
                               strides, rows, cols, additive, stream)
             return
-        if projections.dtype != torch.float32:
+        if _dev.dtype_name(projections) != "float32":
             raise NotImplementedError(
                 "Currently there is only support for dtype float32, but got "
                 f"{projections.dtype}.")
@@ -322,7 +324,6 @@ class RayDrivenConeFP(BaseKernel):
         (angle, row, col) element strides inside an entry.  The table is
         cached per list of addresses, so repeated calls (and graph
         capture) allocate nothing."""
-        import torch
         if len(projections) != expect[0]:
             raise ValueError(
                 "The length of projection array must match the number of "
@@ -331,9 +332,11 @@ class RayDrivenConeFP(BaseKernel):
                 f"{len(projections)}.")
         for p in projections:
             if not is_device_array(p):
-                raise TypeError("Projection data must be device tensors, "
+                raise TypeError("Projection data must be device arrays, "
                                 f"but got {type(p)}.")
-            if p.dtype != torch.float32:
+        projections = [_dev.asarray(p) for p in projections]
+        for p in projections:
+            if _dev.dtype_name(p) != "float32":
                 raise NotImplementedError(
                     "Currently there is only support for dtype float32, "
                     f"but got {p.dtype}.")
@@ -350,8 +353,12 @@ class RayDrivenConeFP(BaseKernel):
                 "All projection files need to reside in own memory.")
         cached = getattr(self, "_ptr_cache", None)
         if cached is None or cached[0] != ptrs:
-            table = torch.tensor(ptrs, dtype=torch.int64).to(
-                projections[0].device)
+            host = np.array(ptrs, dtype=np.int64)
+            if _dev.is_torch(projections[0]):
+                torch = _dev._torch_module()
+                table = torch.from_numpy(host).to(projections[0].device)
+            else:
+                table = _dev.from_host(host, dtype=np.int64)
             self._ptr_cache = (ptrs, table)
         strides = [0, 0, 0]
         strides[self._proj_axs[1]] = projections[0].stride(0)
@@ -401,14 +408,14 @@ class RayDrivenConeFP(BaseKernel):
         if not self._params_are_set:
             raise KernelMemoryUnpreparedException(
                 "Please set the projection geometry with `set_params()`.")
-        import torch
         for name, t in (("projections", projections), ("volume", volume)):
             if not is_device_array(t):
-                raise TypeError(f"`{name}` must be a device tensor.")
-            if t.dtype != torch.float32:
+                raise TypeError(f"`{name}` must be a device array.")
+            if _dev.dtype_name(t) != "float32":
                 raise NotImplementedError(
                     "Currently there is only support for dtype float32, but "
                     f"got {t.dtype}.")
+        projections, volume = _dev.asarray(projections), _dev.asarray(volume)
         n = self._params.shape[1]
         rows, cols = detector_shape(projection_geometry)
         logical = (n, rows, cols)
@@ -439,9 +446,8 @@ class RayDrivenConeFP(BaseKernel):
             self._plan.ptr, rows, cols))
         if n_work < 0:
             raise ValueError(f"bad detector size {rows} x {cols}")
-        work = torch.empty(n_work, dtype=torch.float32,
-                           device=projections.device)
-        if stream is not None:
+        work = _dev.empty((n_work,), like=projections)
+        if stream is not None and _dev.is_torch(work):
             work.record_stream(stream)
         _native.check(_native.lib().kk_cone_fp_transpose(
             self._plan.ptr, _native._vp(projections.data_ptr()),

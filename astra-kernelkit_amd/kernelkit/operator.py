@@ -2,7 +2,9 @@
 
 Drop-in for ``kernelkit/operator.py`` of the reference: ``BaseOperator``
 (:10-33), ``ProjectorOperator`` with its ``.T`` adjoint wrapper (:36-146) and
-``XrayTransform`` (:149-233).  Inputs/outputs are device tensors.
+``XrayTransform`` (:149-233).  Inputs and outputs are device arrays (torch
+tensors, :class:`~kernelkit.DeviceArray` or ``__cuda_array_interface__``
+arrays); an output the operator allocates is of the input's kind.
 """
 from __future__ import annotations
 
@@ -14,10 +16,13 @@ from kernelkit.projector import (BackProjector, BaseProjector,  # noqa: F401
                                  ForwardProjector)
 
 
-def _zeros(shape):
-    import torch
+def _zeros(shape, like=None):
+    """A zero output of the kind of the input ``like`` (a torch tensor for
+    torch inputs, else a :class:`~kernelkit.DeviceArray`)."""
+    from kernelkit import device
     from kernelkit.kernel import default_device
-    return torch.zeros(shape, dtype=torch.float32, device=default_device())
+    default_device()          # raises without a HIP device
+    return device.zeros(shape, like=like)
 
 
 class BaseOperator(ABC):
@@ -67,7 +72,7 @@ class ProjectorOperator(BaseOperator, ABC):
                 f"Input shape {tuple(input.shape)} does not match domain "
                 f"shape {self.domain_shape} of operator.")
         if out is None:
-            out = _zeros(self.range_shape)
+            out = _zeros(self.range_shape, input)
         self.projector.volume = input
         self.projector.projections = out
         self.projector(additive=additive)
@@ -98,7 +103,7 @@ class ProjectorOperator(BaseOperator, ABC):
                         f"Input shape {tuple(input.shape)} does not match "
                         f"domain shape {self.domain_shape} of operator.")
                 if out is None:
-                    out = _zeros(self.range_shape)
+                    out = _zeros(self.range_shape, input)
                 self.projector.projections = input
                 self.projector.volume = out
                 self.projector(additive)
@@ -135,7 +140,7 @@ class ProjectorOperator(BaseOperator, ABC):
                         f"Input shape {tuple(input.shape)} does not match "
                         f"domain shape {self.domain_shape} of operator.")
                 if out is None:
-                    out = _zeros(self.range_shape)
+                    out = _zeros(self.range_shape, input)
                 p = outer.projector
                 p._require_geometry()
                 p.kernel.transpose(input, p._proj_geom_seq,

@@ -104,17 +104,17 @@ def preweight_params(geoms, detector_piercings=None) -> np.ndarray:
 def _as_stack(projections):
     """(tensor, writeback) for a device tensor or list of 2-D tensors."""
     if isinstance(projections, (list, tuple)):
-        import torch
-        stack = torch.stack([as_device_array(p) for p in projections])
+        from kernelkit import device
+        stack = device.stack([as_device_array(p) for p in projections])
 
         def writeback():
             for k, p in enumerate(projections):
                 p.copy_(stack[k])
         return stack, writeback
     if not is_device_array(projections):
-        raise TypeError("projections must be a device tensor or a list of "
-                        "device tensors")
-    return projections, None
+        raise TypeError("projections must be a device array or a list of "
+                        "device arrays")
+    return as_device_array(projections), None
 
 
 def preweight(projections, geoms, detector_piercings=None,

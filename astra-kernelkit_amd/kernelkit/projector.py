@@ -5,7 +5,8 @@ Drop-in for ``kernelkit/projector.py`` of the reference: ``BaseProjector``
 ``BackProjector`` (:349-588), with the same properties, setters, exceptions
 and ``__call__(additive=False)``.
 
-Arrays are PyTorch device tensors (the reference uses CuPy arrays).  Setting
+Arrays are device arrays: PyTorch-ROCm tensors, :class:`~kernelkit.DeviceArray`
+or any ``__cuda_array_interface__`` array (the reference uses CuPy arrays).  Setting
 ``volume`` (FP) or ``projections`` (BP) copies the data into a padded
 texture, like the reference's CUDA-array copy; later in-place changes of the
 source are only seen after setting it again.
@@ -213,7 +214,7 @@ class ForwardProjector(Projector):
     def volume(self, value):
         """Set the volume: copied into the projector's padded texture."""
         if not is_device_array(value):
-            raise TypeError("Volume must be a device tensor.")
+            raise TypeError("Volume must be a device array.")
         self._vol_ref = value
         self._build_texture()
 
@@ -226,7 +227,7 @@ class ForwardProjector(Projector):
         tex = getattr(self, "_texture", None)
         if (tex is None or tex.world_shape != tuple(world)
                 or not tex.covers(mask)):
-            tex = VolumeTexture(world, mask, device=value.device)
+            tex = VolumeTexture(world, mask, like=as_device_array(value))
             self._texture = tex
         tex.update(value, self.volume_axes)
 
@@ -247,8 +248,11 @@ class ForwardProjector(Projector):
     @projections.setter
     def projections(self, value):
         if not (is_device_array(value) or isinstance(value, list)):
-            raise TypeError("Projections must be a device tensor or `list`.")
+            raise TypeError("Projections must be a device array or `list`.")
         if isinstance(value, list):
+            # in place: other libraries' arrays become views of their memory
+            value = [as_device_array(p) if is_device_array(p) else p
+                     for p in value]
             ptrs = [p.data_ptr() for p in value]
             if len(ptrs) != len(set(ptrs)):
                 raise ValueError(
@@ -370,8 +374,8 @@ class BackProjector(Projector):
             if pa[0] != 0:
                 raise ValueError("Can only use a list of projections when "
                                  "projection axis is 0.")
-            import torch
-            value = torch.stack([as_device_array(p) for p in value])
+            from kernelkit import device
+            value = device.stack([as_device_array(p) for p in value])
         elif self._texture_type == "layered" and pa[0] != 0:
             raise ValueError("Can only use layered texture when the first "
                              "projection axis is 0.")

@@ -55,12 +55,10 @@ def fp_samples(pgs, vg) -> int:
     """Bilinear samples of one forward projection: every ray samples each
     slice along its angle's major axis."""
     import numpy as np
-    from kernelkit.geom.proj import ProjectionGeometrySequence
-    import oracle
-    frames = oracle.normalized_frames(pgs, vg)
-    axes = oracle.fp_major_axes(frames)
-    seq = ProjectionGeometrySequence.fromList(list(pgs))
-    rays = int(seq.detector.rows[0]) * int(seq.detector.cols[0])
+    from kernelkit.kernels.cone_fp import RayDrivenConeFP
+    axes = RayDrivenConeFP.geoms2params(list(pgs), vg).ray_axes
+    g = pgs[0]
+    rays = int(g.detector.rows) * int(g.detector.cols)
     shape = np.asarray(vg.shape, dtype=np.int64)
     return int(rays * shape[axes].sum())
 
@@ -196,7 +194,159 @@ def cpu_baseline(pgs, vg, vol, workers=16):
                    f"the {nx}^3 volume ({w_bp:.1f} s wall); scaled to full "
                    f"angles of FP+BP by ray/voxel count"),
         "single_core": single,
+        # wall seconds per angle of the full workload on `workers` cores
+        "fp_s_per_angle": round(w_fp * rows / band / workers, 4),
+        "bp_s_per_angle": round(w_bp * nx / slab / workers, 4),
     }
+
+
+def _small_task(args):
+    """One host process of the small-workload baseline (one core): ``fp``
+    projects the shared volume into its angles of the shared projection
+    array; ``bp`` backprojects its angles of the shared projections into
+    its own slot of the shared partial volumes."""
+    import numpy as np
+    from multiprocessing import shared_memory
+    from threadpoolctl import threadpool_limits
+
+    import oracle
+
+    kind, names, shapes, geoms, a0, slot, vg = args
+    if kind == "start":        # imports done, process up
+        return 0
+    shms = [shared_memory.SharedMemory(name=n) for n in names]
+    try:
+        vol, proj, parts = (np.ndarray(sh, np.float64, buffer=m.buf)
+                            for m, sh in zip(shms, shapes))
+        with threadpool_limits(limits=1):
+            if kind == "fp":
+                proj[a0:a0 + len(geoms)] = oracle.cone_fp(vol, geoms, vg)
+            else:
+                parts[slot] = oracle.cone_bp(proj[a0:a0 + len(geoms)],
+                                             geoms, vg)
+        del vol, proj, parts
+    finally:
+        for m in shms:
+            m.close()
+    return 0
+
+
+class _SmallPool:
+    """FP64 oracle FP/BP over angle chunks on ``workers`` host processes
+    (one core each); FP and BP are sums over angles, so the chunks are
+    independent and their results add up exactly."""
+
+    def __init__(self, pgs, vg, workers):
+        import multiprocessing as mp
+        from concurrent.futures import ProcessPoolExecutor
+        from multiprocessing import shared_memory
+
+        import numpy as np
+        self.pgs, self.vg, self.workers = list(pgs), vg, workers
+        n = len(self.pgs)
+        det = (int(self.pgs[0].detector.rows), int(self.pgs[0].detector.cols))
+        vshape = tuple(int(s) for s in vg.shape)
+        self.shapes = [vshape, (n,) + det, (workers,) + vshape]
+        self.shms = [shared_memory.SharedMemory(
+            create=True, size=8 * int(np.prod(sh))) for sh in self.shapes]
+        self.arrays = [np.ndarray(sh, np.float64, buffer=m.buf)
+                       for m, sh in zip(self.shms, self.shapes)]
+        bounds = np.linspace(0, n, workers + 1).round().astype(int)
+        self.chunks = [(int(a), int(b)) for a, b in zip(bounds[:-1],
+                                                        bounds[1:]) if b > a]
+        self.pool = ProcessPoolExecutor(workers, mp_context=mp.get_context(
+            "spawn"))
+
+    def _run(self, kind):
+        names = [m.name for m in self.shms]
+        tasks = [(kind, names, self.shapes, self.pgs[a:b], a, k, self.vg)
+                 for k, (a, b) in enumerate(self.chunks)]
+        list(self.pool.map(_small_task, tasks, timeout=3600))
+
+    def fp(self, x):
+        self.arrays[0][...] = x
+        self._run("fp")
+        return self.arrays[1].copy()
+
+    def bp(self, p):
+        self.arrays[1][...] = p
+        self.arrays[2][...] = 0.0
+        self._run("bp")
+        return self.arrays[2][:len(self.chunks)].sum(axis=0)
+
+    def close(self):
+        self.pool.shutdown()
+        del self.arrays
+        for m in self.shms:
+            m.close()
+            m.unlink()
+
+
+def _small_run(pool, vol, y, sirt_iters):
+    """FP, BP, FDK (Ram-Lak) and ``sirt_iters`` SIRT iterations through the
+    pool, restating the oracle's FDK and SIRT with the pool's FP/BP.
+    Returns seconds per part and the SIRT result."""
+    import numpy as np
+
+    import oracle
+    from oracle.algorithms import _safe_reciprocal
+
+    t = {}
+    t0 = time.perf_counter()
+    pool.fp(vol)
+    t["fp"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pool.bp(y)
+    t["bp"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    pool.bp(oracle.filter_rows(oracle.preweight(y, pool.pgs), "ramlak"))
+    t["fdk"] = time.perf_counter() - t0
+    x = None
+    if sirt_iters:
+        t0 = time.perf_counter()
+        x = np.zeros(vol.shape)
+        C = _safe_reciprocal(pool.bp(np.ones_like(y)))
+        R = _safe_reciprocal(pool.fp(np.ones(vol.shape)))
+        for _ in range(sirt_iters):
+            x = x - C * pool.bp((pool.fp(x) - y) * R)
+        t["sirt"] = time.perf_counter() - t0
+    return t, x
+
+
+def cpu_small(workers=16, sirt_iters=20, single_sirt_iters=1):
+    """CPU baseline at the small workload (128^3, 180 angles, 160^2
+    detector) in full: FP, BP, Ram-Lak FDK and ``sirt_iters`` SIRT
+    iterations of the FP64 oracle on ``workers`` host processes, and the
+    same on one process (SIRT there for ``single_sirt_iters`` iterations,
+    with their setup: the two normalisation passes)."""
+    import numpy as np
+
+    from kernelkit import workloads
+
+    pgs, vg = workloads.config_geometry("small")
+    vol = workloads.ellipsoid_phantom(vg.shape, n=10, seed=0).astype(
+        np.float64)
+    import oracle
+    y = oracle.cone_fp(vol, pgs[:1], vg)  # warm the imports
+    del y
+    out = {"workload": "128^3 volume, 180 angles, 160^2 detector",
+           "kind": "port"}
+    workers = max(1, min(workers, os.cpu_count() or 1))
+    for cores, iters in ((workers, sirt_iters), (1, single_sirt_iters)):
+        pool = _SmallPool(pgs, vg, cores)
+        try:
+            list(pool.pool.map(_small_task, [("start",) + (None,) * 6] *
+                               cores))   # start the workers
+            y = pool.fp(vol)
+            t, _ = _small_run(pool, vol, y, iters)
+        finally:
+            pool.close()
+        out[f"{cores}_cores"] = {
+            "fp_s": round(t["fp"], 2), "bp_s": round(t["bp"], 2),
+            "fdk_s": round(t["fdk"], 2),
+            f"sirt_{iters}_iterations_s": round(t["sirt"], 2)}
+        _log(f"cpu small, {cores} cores: {out[f'{cores}_cores']}")
+    return out
 
 
 def l2_read_rate(stream, mib=2, blocks=2048, iters=1024):
@@ -218,6 +368,44 @@ def l2_read_rate(stream, mib=2, blocks=2048, iters=1024):
     run()
     ms = min(_event_ms(run, stream) for _ in range(5))
     return blocks * 256 * 8 * iters * 16 / (ms * 1e-3) / 1e9
+
+
+# (name, footprint_log2 in float2, lane step across, lane step down, iters):
+# the projectors' tile (neighbouring rays read neighbouring texels: lane
+# step 1 float2 across, one frame row of 512 + 3 texels down) and a fully
+# coalesced tile (64 contiguous 16-byte loads), each L1-resident (16 KiB),
+# L2-resident (2 MiB) and HBM-resident (1 GiB)
+GATHER_PROBES = [
+    ("tile_l1", 11, 1, 515, 1024), ("coalesced_l1", 11, 2, 16, 1024),
+    ("tile_l2", 18, 1, 515, 256), ("coalesced_l2", 18, 2, 16, 256),
+    ("tile_hbm", 27, 1, 515, 8), ("coalesced_hbm", 27, 2, 16, 8),
+]
+
+
+def gather_ceilings(stream, blocks=4096):
+    """Read rates (GB/s) of ``kk_probe_gather``: 16-byte loads in 64-lane
+    8 x 8 tiles at pseudo-random bases, the access shape of the projectors'
+    texture gathers, over footprints resident in L1, L2 and HBM."""
+    import torch
+
+    from kernelkit import _native
+    n2 = (1 << 27) + 1
+    buf = torch.ones(2 * n2, dtype=torch.float32, device="cuda")
+    out = torch.empty(blocks * 256, dtype=torch.float32, device="cuda")
+    lib = _native.lib()
+    rates = {}
+    for name, k, sx, sy, iters in GATHER_PROBES:
+        def run():
+            _native.check(lib.kk_probe_gather(
+                _native._vp(buf.data_ptr()), buf.numel(), k, sx, sy, blocks,
+                iters, _native._vp(out.data_ptr()),
+                _native.stream_ptr(stream)))
+        run()
+        ms = min(_event_ms(run, stream) for _ in range(3))
+        rates[name] = round(blocks * 256 * 8 * iters * 16
+                            / (ms * 1e-3) / 1e9, 1)
+    del buf, out
+    return rates
 
 
 def _event_ms(fn, stream, reps=1):
@@ -342,11 +530,16 @@ def algorithms_full(pgs, vg, x, iters=50):
             "sirt_ms_per_iteration": round(t_sirt / iters, 2)}
 
 
-def autograd_diff():
+def autograd_diff(ceiling=None):
     """Forward + backward of ||A x - y||^2 through the exact transpose
     against two plain calls (forward projection + backprojection) at 256^3,
     360 angles, 384^2 detector; also through the reference's default
-    autograd operator (voxel-driven backprojector in the backward pass)."""
+    autograd operator (voxel-driven backprojector in the backward pass).
+
+    The transpose moves the forward projection's samples backwards: each
+    sample's four tap weights are gathered by the voxels they touch, 16 B
+    per sample as in the forward gather, so its rate against the gather
+    ``ceiling`` (GB/s) compares directly with the forward projection's."""
     import numpy as np
     import torch
 
@@ -394,7 +587,7 @@ def autograd_diff():
     yw = Aw(x)
     t_fpw = _event_ms(lambda: Aw(x, out=yw), stream, reps=3)
     t_tw = _event_ms(lambda: Aw.matched_T(yw, out=v), stream, reps=3)
-    return {
+    res = {
         "workload": "256^3 volume, 360 angles, 384^2 detector",
         "plain_fp_plus_bp_ms": round(t_plain, 3),
         "autograd_fwd_bwd_ms": round(t_ag, 3),
@@ -409,6 +602,18 @@ def autograd_diff():
         "wobble_transpose_ms": round(t_tw, 3),
         "wobble_transpose_vs_fp": round(t_tw / t_fpw, 3),
     }
+    if ceiling:
+        b = 16 * fp_samples(pgs, vg)
+        bw = 16 * fp_samples(pgw, vg)
+        res.update({
+            "gather_ceiling_GBs": ceiling,
+            "fp_frac": round(b / (t_fp * 1e-3) / 1e9 / ceiling, 4),
+            "transpose_frac": round(b / (t_t * 1e-3) / 1e9 / ceiling, 4),
+            "wobble_fp_frac": round(bw / (t_fpw * 1e-3) / 1e9 / ceiling, 4),
+            "wobble_transpose_frac": round(
+                bw / (t_tw * 1e-3) / 1e9 / ceiling, 4),
+        })
+    return res
 
 
 def large_unrolled(iters=5):
@@ -510,8 +715,9 @@ def main(argv=None):
     ap.add_argument("--extras", default="batched,autograd,algorithms",
                     help="comma-separated extra measurements after the "
                          "FP/BP step at the full config: batched, "
-                         "autograd, algorithms, large (not in the "
-                         "default list: several minutes)")
+                         "autograd, algorithms, large, cpu_small (the "
+                         "last two not in the default list: several "
+                         "minutes each)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     metavar="MIB",
                     help="after the timed steps, run one plain device copy "
@@ -601,6 +807,9 @@ def main(argv=None):
         mean(3, 4)
     l2_rate = l2_read_rate(stream)
     _log(f"L2 read probe: {l2_rate:.0f} GB/s")
+    gathers = gather_ceilings(stream)
+    ceiling = max(v for k, v in gathers.items() if k.endswith("_l1"))
+    _log(f"gather probes (GB/s): {gathers}")
     ms_step = wall * 1e3 / args.steps
     updates = int(np.prod(shape)) * n
     value = 2 * updates / (ms_step * 1e-3) / 1e9
@@ -639,19 +848,24 @@ def main(argv=None):
             "bp_texture_ms": round(t_btex, 3),
         },
         "roofline": {
-            "bound": "l2",
+            "bound": "gather",
             "kernel": "cone_fp_kernel",
             "achieved": round(achieved, 1),
-            "peak": round(L2_PEAK_GBS, 1),
+            # measured in this run: the best rate of 16-byte gathers in
+            # the projectors' access shape from an L1-resident footprint
+            "peak": ceiling,
+            "peak_source": "kk_probe_gather, best L1-resident shape",
             "unit": "GB/s",
-            "frac": round(achieved / L2_PEAK_GBS, 4),
+            "frac": round(achieved / ceiling, 4),
             "traffic": traffic,
             "algorithmic_bytes": fp_bytes,
             "bytes_per_update": round(fp_bytes / updates, 4),
+            "gather_probes_GBs": gathers,
             "hbm_compulsory_GBs": round(
                 compulsory_fp / (t_fp * 1e-3) / 1e9, 1),
-            "l2_measured_GBs": round(l2_rate, 1),
-            "frac_of_measured_l2": round(achieved / l2_rate, 4),
+            "l2_stream_GBs": round(l2_rate, 1),
+            "l2_theoretical_GBs": round(L2_PEAK_GBS, 1),
+            "frac_of_l2_theoretical": round(achieved / L2_PEAK_GBS, 4),
             "hbm_peak": HBM_PEAK_GBS,
             # measured beyond-L2 (fabric/HBM) bytes of the launch at this
             # launch's time, against the HBM peak
@@ -662,7 +876,7 @@ def main(argv=None):
                 round(traffic / (t_fp * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)),
             "bp_kernel": {
                 "achieved": round(bp_bytes / (t_bp * 1e-3) / 1e9, 1),
-                "frac": round(bp_bytes / (t_bp * 1e-3) / 1e9 / L2_PEAK_GBS,
+                "frac": round(bp_bytes / (t_bp * 1e-3) / 1e9 / ceiling,
                               4),
                 "bytes_per_update": 16,
             },
@@ -671,7 +885,8 @@ def main(argv=None):
     }
     extras = set() if args.quick or args.config != "full" else {
         e.strip() for e in args.extras.split(",") if e.strip()}
-    unknown = extras - {"batched", "autograd", "algorithms", "large"}
+    unknown = extras - {"batched", "autograd", "algorithms", "large",
+                        "cpu_small"}
     if unknown:
         print(f"unknown extras: {sorted(unknown)}", file=sys.stderr)
         return 2
@@ -683,7 +898,7 @@ def main(argv=None):
         out["batched_small"] = batched_sirt(scale=0.25)
     if "autograd" in extras:
         _log("autograd (exact transpose) vs plain ...")
-        out["autograd"] = autograd_diff()
+        out["autograd"] = autograd_diff(ceiling)
     if "algorithms" in extras:
         _log("FDK and SIRT at full size ...")
         out["algorithms"] = algorithms_full(pgs, vg, x)
@@ -694,7 +909,26 @@ def main(argv=None):
         out["large"] = large_unrolled()
     if not args.no_cpu_baseline:
         _log("cpu baseline (FP64 oracle, 1 core and 16 processes) ...")
-        out["cpu_baseline"] = cpu_baseline(pgs, vg, vol)
+        cb = out["cpu_baseline"] = cpu_baseline(pgs, vg, vol)
+        if "algorithms" in out:
+            # the full-size FDK and SIRT on the same host cores, from the
+            # sampled per-angle FP and BP times (the filter is left out)
+            alg = out["algorithms"]
+            t_bp_cpu = n * cb["bp_s_per_angle"]
+            t_it_cpu = n * (cb["fp_s_per_angle"] + cb["bp_s_per_angle"])
+            alg["cpu_estimate"] = {
+                "cores": cb["cores"],
+                "fdk_s": round(t_bp_cpu, 1),
+                "sirt_s_per_iteration": round(t_it_cpu, 1),
+                "fdk_gpu_speedup": round(t_bp_cpu * 1e3 /
+                                         alg["fdk_hann_ms"], 1),
+                "sirt_gpu_speedup": round(
+                    t_it_cpu * 1e3 / alg["sirt_ms_per_iteration"], 1),
+                "basis": "cpu_baseline's per-angle FP/BP wall times x "
+                         f"{n} angles"}
+    if "cpu_small" in extras:
+        _log("cpu baseline at the small workload, in full ...")
+        out["cpu_small"] = cpu_small()
     print(json.dumps(out), flush=True)
     return 0
 

@@ -105,6 +105,31 @@ int kk_projections_to_bp_texture(const float* proj, int32_t n_angles,
 /* value-fill of n floats (a kernel, not hipMemsetAsync: graph-safe). */
 int kk_fill(float* ptr, int64_t n, float value, void* stream);
 
+/* ---- device memory without PyTorch ---------------------------------- */
+
+/* The reference's arrays are CuPy arrays (kernelkit/kernel.py:47-119,
+ * kernelkit/projector.py:266-283): CuPy allocates, copies and synchronises
+ * for it.  These calls give the drop-in (kernelkit.DeviceArray) and C
+ * callers the same without PyTorch.  kk_memcpy is asynchronous on the
+ * stream (host memory must stay valid until the stream is synchronised);
+ * kk_copy_strided copies a 3-D index space of shape[0] x shape[1] x
+ * shape[2] elements of elem_bytes (4 or 8) between any element strides, on
+ * the device. */
+#define KK_COPY_H2D 1
+#define KK_COPY_D2H 2
+#define KK_COPY_D2D 3
+int kk_device_count(int32_t* count);
+int kk_get_device(int32_t* device);
+int kk_set_device(int32_t device);
+int kk_malloc(void** ptr, int64_t bytes);
+int kk_free(void* ptr);
+int kk_memcpy(void* dst, const void* src, int64_t bytes, int32_t kind,
+              void* stream);
+int kk_stream_synchronize(void* stream);
+int kk_copy_strided(const void* src, const int64_t src_strides[3], void* dst,
+                    const int64_t dst_strides[3], const int64_t shape[3],
+                    int32_t elem_bytes, void* stream);
+
 /* Ray-driven Joseph forward projection, all angles in one launch
  * (cone_fp<DIR_X|Y|Z>, kernelkit/cuda/cone_fp.cu:73-207, launched by
  * kernels/cone_fp.py:195-262).  vol_tex: FP texture holding at least the
@@ -342,6 +367,19 @@ int kk_filter_rows_to_bp_texture(const kk_filter* filter, const float* in,
  * * 16). */
 int kk_probe_l2_read(const float* buf, int64_t n_floats, int32_t blocks,
                      int32_t iters, float* out, void* stream);
+
+/* No counterpart in the reference.  The projectors' access shape: each
+ * thread of `blocks` workgroups of 256 issues 8 * iters 16-byte loads at
+ * 8-byte alignment (two neighbouring float2); the 64 lanes of a wave form
+ * an 8 x 8 tile with lane steps sx and sy (float2 units, in [0, 2^20]), and
+ * every load moves the tile to a pseudo-random base inside the first
+ * 2^footprint_log2 float2 of buf (footprint_log2 in [6, 30]; n_floats >=
+ * 2 * (2^footprint_log2 + 1), buf 8-byte aligned).  Writes one float per
+ * thread to out (blocks * 256 floats).  Bytes read = blocks * 256 * 8 *
+ * iters * 16. */
+int kk_probe_gather(const float* buf, int64_t n_floats,
+                    int32_t footprint_log2, int32_t sx, int32_t sy,
+                    int32_t blocks, int32_t iters, float* out, void* stream);
 
 #ifdef __cplusplus
 }
