@@ -130,10 +130,12 @@ __device__ __forceinline__ bool project(const kk::FpAngle& P, double xa,
   const double vq0 = P.vb * P.qc - P.vc * P.qb;
   const double vq1 = P.vc * P.qa - P.va * P.qc;
   const double vq2 = P.va * P.qb - P.vb * P.qa;
-  const double den = P.ua * vx0 + P.ub * vx1 + P.uc * vx2;
-  col = -(P.qa * vx0 + P.qb * vx1 + P.qc * vx2) / den;
-  row = -(P.ua * qx0 + P.ub * qx1 + P.uc * qx2) / den;
-  const double lam = (P.ua * vq0 + P.ub * vq1 + P.uc * vq2) / den;
+  // one reciprocal for the three quotients (a box corner: its float64
+  // rounding is far below the 1e-6 voxel widening of the box)
+  const double inv = 1.0 / (P.ua * vx0 + P.ub * vx1 + P.uc * vx2);
+  col = -(P.qa * vx0 + P.qb * vx1 + P.qc * vx2) * inv;
+  row = -(P.ua * qx0 + P.ub * qx1 + P.uc * qx2) * inv;
+  const double lam = (P.ua * vq0 + P.ub * vq1 + P.uc * vq2) * inv;
   return isfinite(col) && isfinite(row) && lam > 0.0;
 }
 
@@ -153,6 +155,29 @@ __device__ __forceinline__ void int_range(float lo, float hi, int32_t n,
   b = min(n - 1, static_cast<int32_t>(floorf(hi)));
 }
 
+// A sample at c offset x from the centre of a lane's first voxel, carrying
+// value y: adds y times the tents of the lane's two voxels, 1 - |x| and
+// 1 - |x - 1|, written max(0, 1 - |x|) and med3(0, x, 2 - x) (exact for
+// x in [0, 2]).
+static_assert(kJC == 2, "tent_pair covers two c voxels per lane");
+__device__ __forceinline__ void tent_pair(float x, float y, float* colsum) {
+  colsum[0] = fmaf(y, fmaxf(0.f, 1.f - fabsf(x)), colsum[0]);
+  colsum[1] = fmaf(y, __builtin_amdgcn_fmed3f(0.f, x, 2.f - x), colsum[1]);
+}
+
+// Adds a run of samples with b tap index t (in [-1, kJB)) to the lane's
+// b-tap sums: S - T to row t and T to row t + 1, where they are inside the
+// b-run; clears the run.
+__device__ __forceinline__ void flush_run(float (*g)[kJC][64], int32_t lane,
+                                          int32_t t, float* S, float* T) {
+#pragma unroll
+  for (int cc = 0; cc < kJC; ++cc) {
+    if (t >= 0) g[t][cc][lane] += S[cc] - T[cc];
+    if (t + 1 < kJB) g[t + 1][cc][lane] += T[cc];
+    S[cc] = T[cc] = 0.f;
+  }
+}
+
 // Value of lane `l` of a wavefront (l wave-uniform), in scalar registers.
 __device__ __forceinline__ int32_t bcast(int32_t v, int32_t l) {
   return __builtin_amdgcn_readlane(v, l);
@@ -170,6 +195,14 @@ __device__ __forceinline__ double bcast(double v, int32_t l) {
   return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
+// Paths a launch compiles (its plan's angles need): the column-coherent
+// one, the general one, or both.  A launch without column-coherent angles
+// does not reserve the column records' LDS, which keeps a sixth workgroup
+// per CU resident.
+constexpr int kPathColumn = 1;
+constexpr int kPathGeneral = 2;
+
+template <int PATHS>
 __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
     const kk::FpAngle* __restrict__ tab, FptArgs A) {
   int64_t bid = blockIdx.x;
@@ -189,15 +222,25 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
   const float kf = static_cast<float>(k);
   const int32_t cstride = A.rows + 2 * kRowPad;  // workspace column stride
   const int64_t angle_elems = static_cast<int64_t>(cstride) * A.cols;
-  // b-tap sums of this lane's voxels at t = -1 .. kJB (row t + 1; rows 0
-  // and kJB + 1 catch taps just outside the b-run and are discarded),
-  // indexed per lane: every lane owns its words, so no atomics and no
-  // cross-lane ordering are needed.  (A register array indexed by the
-  // wave-uniform tap row was compiled into private scratch memory.)
-  __shared__ float gacc[kWaves][kJB + 2][kJC][64];
+  // b-tap sums of this lane's voxels (row t: b voxel jw + t), indexed per
+  // lane: every lane owns its words, so no atomics and no cross-lane
+  // ordering are needed; taps just outside the b-run are not added.  (A
+  // register array indexed by the wave-uniform tap row was compiled into
+  // private scratch memory.)
+  __shared__ float gacc[kWaves][kJB][kJC][64];
   float(*g)[kJC][64] = gacc[threadIdx.x >> 6];
+  // column records of the column-coherent path, written lane-parallel and
+  // read back per column at a wave-uniform address (an LDS broadcast, which
+  // costs no vector-ALU slot, unlike a readlane): {alpha, beta} in float64
+  // and {alpha, 1 / alpha, window origin, b fraction} in float32.  Each
+  // wavefront uses its own records, in program order: no barrier.
+  constexpr int kRec = (PATHS & kPathColumn) != 0 ? 64 : 1;
+  __shared__ double2 crd_s[kWaves][kRec];
+  __shared__ float4 crf_s[kWaves][kRec];
+  double2* crd = crd_s[threadIdx.x >> 6];
+  float4* crf = crf_s[threadIdx.x >> 6];
 #pragma unroll
-  for (int t = 0; t < kJB + 2; ++t)
+  for (int t = 0; t < kJB; ++t)
 #pragma unroll
     for (int cc = 0; cc < kJC; ++cc) g[t][cc][lane] = 0.f;
 
@@ -206,7 +249,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
     const kk::FpAngle& P = tab[ang];
     const float* img = A.yw + ang * angle_elems + kRowPad;
     const double arel = fi + P.sig;  // slice centre, relative to the source
-    if (P.colc) {
+    if (P.colc) { if constexpr ((PATHS & kPathColumn) != 0) {
       // columns whose b coordinate at this slice lies in
       // [jw, jw + kJB + 1): a1 = (x_b - taub) / arel, and for these
       // angles a1 = (col ub + qb) / (col ua + qa)
@@ -220,6 +263,9 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         if (isfinite(c1) && isfinite(c2))
           int_range(fmin(c1, c2), fmax(c1, c2), A.cols, c_lo, c_hi);
       }
+      // computed from wave-uniform values: scalar loop control below
+      c_lo = __builtin_amdgcn_readfirstlane(c_lo);
+      c_hi = __builtin_amdgcn_readfirstlane(c_hi);
       for (int32_t cb = c_lo; cb <= c_hi; cb += 64) {
         const int32_t ncol = min(64, c_hi - cb + 1);
         // ray data of column cb + lane (kk_fp_ray's terms: the row terms
@@ -227,7 +273,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         // of the column's rays at this slice is affine in the row:
         // x_c(r) = beta + alpha r (padded coordinates, float64).
         double alpha = 0.0, beta = 0.0;
-        float fb = 0.f, ia = 0.f, betaf = 0.f, alphaf = 0.f, off = 0.f;
+        float fb = 0.f, ia = 0.f, alphaf = 0.f, off = 0.f;
         int32_t t0 = -99, nrow = 0;
         if (lane < ncol) {
           const double fc = static_cast<double>(cb + lane);
@@ -247,7 +293,6 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
           if (!isfinite(alpha) || !isfinite(beta)) t0 = -99;
           alphaf = static_cast<float>(alpha);
           ia = fabsf(alphaf) > 1e-9f ? 1.f / alphaf : 0.f;
-          betaf = static_cast<float>(beta);
           // a lane's window: the rows whose c coordinate lies in the open
           // padded span (k, k + kJC + 1) of its voxels' tents, (kJC + 1)
           // |1 / alpha| rows long, i.e. at most floor(len) + 1 integer
@@ -261,14 +306,20 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
             off = fminf(0.f, static_cast<float>(kJC + 1) * ia) - 1e-3f;
           }
         }
+        // window origin: the lane's first row is k / alpha + origin
+        const float org = fmaf(-static_cast<float>(beta), ia, off);
+        crd[lane] = make_double2(alpha, beta);
+        crf[lane] = make_float4(alphaf, ia, org, fb);
         // the column's tap row and window length, one broadcast
         const int32_t tpk = (min(max(t0, -99), 100) + 128) | (nrow << 8);
         for (int32_t m = 0; m < ncol; ++m) {
           const int32_t pk = bcast(tpk, m);
           const int32_t tm = (pk & 255) - 128;
           if (tm < -1 || tm >= kJB) continue;
-          const float iam = bcast(ia, m);
           const int32_t nr = pk >> 8;
+          const float4 cf = crf[m];
+          const double2 cd = crd[m];
+          const float alm = cf.x, iam = cf.y;
           const float* ycol = img + (cb + m) * cstride;
           float colsum[kJC];
 #pragma unroll
@@ -277,8 +328,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
             // nr rows (wave-uniform: no per-lane loop control) from the
             // ceiling of the window's first row, clamped into the padded
             // column (a window off the detector reads zeros)
-            const float r1 =
-                fmaf(kf - bcast(betaf, m), iam, bcast(off, m));
+            const float r1 = fmaf(kf, iam, cf.z);
             const int32_t r_lo =
                 min(max(static_cast<int32_t>(ceilf(r1)), -kRowPad),
                     A.rows + kRowPad - nr);
@@ -288,10 +338,8 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
             // so the offsets carry ~1e-7 voxel of rounding).  A voxel's
             // weight is the tent 1 - |x_c - centre| of the sample, equal
             // to the forward kernel's bilinear tap weight.
-            const float alm = bcast(alphaf, m);
             const float loc = static_cast<float>(
-                fma(bcast(alpha, m), static_cast<double>(r_lo),
-                    bcast(beta, m)) - (kd + 1.0));
+                fma(cd.x, static_cast<double>(r_lo), cd.y) - (kd + 1.0));
             const float* yw = ycol + r_lo;
             int32_t u = 0;
             float uf = 0.f;   // u as a float, counted (exact)
@@ -299,34 +347,21 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
               float y[kRU];
 #pragma unroll
               for (int t = 0; t < kRU; ++t) y[t] = yw[u + t];
+              const float xg = fmaf(alm, uf, loc);
 #pragma unroll
-              for (int t = 0; t < kRU; ++t) {
-                const float x =
-                    fmaf(alm, uf + static_cast<float>(t), loc);
-#pragma unroll
-                for (int cc = 0; cc < kJC; ++cc)
-                  colsum[cc] = fmaf(
-                      y[t],
-                      fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
-                      colsum[cc]);
-              }
+              for (int t = 0; t < kRU; ++t)
+                tent_pair(t == 0 ? xg : fmaf(alm, static_cast<float>(t), xg),
+                          y[t], colsum);
             }
-            for (; u < nr; ++u, uf += 1.f) {
-              const float yv = yw[u];
-              const float x = fmaf(alm, uf, loc);
-#pragma unroll
-              for (int cc = 0; cc < kJC; ++cc)
-                colsum[cc] = fmaf(
-                    yv, fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
-                    colsum[cc]);
-            }
+            for (; u < nr; ++u, uf += 1.f)
+              tent_pair(fmaf(alm, uf, loc), yw[u], colsum);
           } else {
           // rows whose c coordinate lies in [k, k + kJC + 1) (the padded
           // span of the lane's voxels' tents), found in float32 and
           // widened by 1e-2 row; samples outside get zero weight below
           int32_t r_lo = 0, r_hi = A.rows - 1;
           if (iam != 0.f) {
-            const float r1 = (kf - bcast(betaf, m)) * iam;
+            const float r1 = fmaf(kf, iam, cf.z);
             const float r2 = fmaf(static_cast<float>(kJC + 1), iam, r1);
             int_range(fminf(r1, r2) - 1e-2f, fmaxf(r1, r2) + 1e-2f, A.rows,
                       r_lo, r_hi);
@@ -334,50 +369,29 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
           // the lane's first row in float64 (the first voxel's centre is
           // at padded coordinate k + 1), float32 offsets after it, tents
           // as above; rays the forward kernel skips have yw = 0
-          const float alm = bcast(alphaf, m);
           const float loc = static_cast<float>(
-              fma(bcast(alpha, m), static_cast<double>(r_lo),
-                  bcast(beta, m)) - (kd + 1.0));
-          int32_t r = r_lo;
+              fma(cd.x, static_cast<double>(r_lo), cd.y) - (kd + 1.0));
           float du = 0.f;
-          for (; r + (kRU - 1) <= r_hi; r += kRU) {
-            float y[kRU];
-#pragma unroll
-            for (int u = 0; u < kRU; ++u) y[u] = ycol[r + u];
-#pragma unroll
-            for (int u = 0; u < kRU; ++u) {
-              const float x = fmaf(alm, du + static_cast<float>(u), loc);
-#pragma unroll
-              for (int cc = 0; cc < kJC; ++cc)
-                colsum[cc] = fmaf(
-                    y[u], fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
-                    colsum[cc]);
-            }
-            du += static_cast<float>(kRU);
+          for (int32_t r = r_lo; r <= r_hi; ++r, du += 1.f)
+            tent_pair(fmaf(alm, du, loc), ycol[r], colsum);
           }
-          for (; r <= r_hi; ++r) {
-            const float yv = ycol[r];
-            const float x = fmaf(alm, du, loc);
+          // the column's two b taps tm, tm + 1 (wave-uniform, so the
+          // tests are scalar branches): read-modify-write of the lane's
+          // own LDS words
+          const float fbm = cf.w;
+          if (tm >= 0) {
 #pragma unroll
             for (int cc = 0; cc < kJC; ++cc)
-              colsum[cc] = fmaf(
-                  yv, fmaxf(0.f, 1.f - fabsf(x - static_cast<float>(cc))),
-                  colsum[cc]);
-            du += 1.f;
+              g[tm][cc][lane] = fmaf(1.f - fbm, colsum[cc], g[tm][cc][lane]);
           }
-          }
-          // the column's two b taps (tm wave-uniform): read-modify-write
-          // of the lane's own LDS words
-          const float fbm = bcast(fb, m);
+          if (tm + 1 < kJB) {
 #pragma unroll
-          for (int cc = 0; cc < kJC; ++cc) {
-            g[tm + 1][cc][lane] = fmaf(1.f - fbm, colsum[cc],
-                                       g[tm + 1][cc][lane]);
-            g[tm + 2][cc][lane] = fmaf(fbm, colsum[cc], g[tm + 2][cc][lane]);
+            for (int cc = 0; cc < kJC; ++cc)
+              g[tm + 1][cc][lane] = fmaf(fbm, colsum[cc], g[tm + 1][cc][lane]);
           }
         }
       }
-    } else {
+    } } else if constexpr ((PATHS & kPathGeneral) != 0) {
       // pixels whose rays meet the support rectangle of this lane's voxels
       // at this slice: bounding box of its projected corners
       double cmin = 1e300, cmax = -1e300, rmin = 1e300, rmax = -1e300;
@@ -430,66 +444,54 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       const float vaf = static_cast<float>(P.va), vbf = static_cast<float>(P.vb),
                   vcf = static_cast<float>(P.vc);
       const bool anchor_ok = ok && isfinite(tb) && isfinite(tcv);
+      // With u the row offset from the lane's first row and dc the column
+      // offset from the wave's first column, dA = dc ua + u va (b, c
+      // alike) and x_b = ob + arel (dB - tb dA) / (Da0 + dA): per column
+      // the numerator is affine in u with slope vb - tb va, and the
+      // denominator with slope va (c alike).
+      const float kvb = fmaf(-tbf, vaf, vbf), kvc = fmaf(-tcf, vaf, vcf);
       for (int32_t col = wc_lo; col <= wc_hi; ++col) {
         const float* ycol = img + col * cstride;
         const float dcf = static_cast<float>(col - wc_lo);
-        const float dAc = dcf * uaf, dBc = dcf * ubf, dCc = dcf * ucf;
+        const float dAc = dcf * uaf;
+        const float nb0 = fmaf(-tbf, dAc, dcf * ubf);
+        const float nc0 = fmaf(-tcf, dAc, dcf * ucf);
+        const float den0 = da0 + dAc;
         const int32_t rl = r_lo;
         const int32_t rh = (anchor_ok && col >= c_lo && col <= c_hi) ? r_hi
                                                                     : r_lo - 1;
         // b-tap sums over a run of samples with the same tap index, added
         // to the lane's LDS words when the index changes (along a column
-        // it rarely does)
+        // it rarely does): S = sum of y w_c, T = sum of y w_c fb, so the
+        // taps t and t + 1 receive S - T and T
         int32_t tcur = -2;  // no run yet
-        float s0[kJC], s1[kJC];
+        float S[kJC], T[kJC];
 #pragma unroll
-        for (int cc = 0; cc < kJC; ++cc) s0[cc] = s1[cc] = 0.f;
+        for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
         float drf = 0.f;
         for (int32_t r = rl; r <= rh; ++r, drf += 1.f) {
-          const float dA = fmaf(drf, vaf, dAc);
-          const float dB = fmaf(drf, vbf, dBc);
-          const float dC = fmaf(drf, vcf, dCc);
-          const float sc = arf * __builtin_amdgcn_rcpf(da0 + dA);
-          const float xb = fmaf(fmaf(-tbf, dA, dB), sc, ob);
-          const float xc = fmaf(fmaf(-tcf, dA, dC), sc, oc);
-          float wk[kJC];
-          bool any = false;
-#pragma unroll
-          for (int cc = 0; cc < kJC; ++cc) {
-            wk[cc] = fmaxf(0.f, 1.f - fabsf(xc - static_cast<float>(cc)));
-            any = any || wk[cc] > 0.f;
-          }
-          if (!any) continue;
+          const float sc = arf * __builtin_amdgcn_rcpf(fmaf(drf, vaf, den0));
+          const float xc = fmaf(fmaf(drf, kvc, nc0), sc, oc);
+          // the lane's tents are zero outside x_c in (-1, kJC)
+          if (!(fabsf(xc - 0.5f) < 1.5f)) continue;
+          const float xb = fmaf(fmaf(drf, kvb, nb0), sc, ob);
           const float fl = floorf(xb);
           if (!(fl >= -1.f && fl < static_cast<float>(kJB))) continue;
           const int32_t t0 = static_cast<int32_t>(fl);
-          const float fb = xb - fl;
           if (t0 != tcur) {
-            if (tcur >= -1) {
-#pragma unroll
-              for (int cc = 0; cc < kJC; ++cc) {
-                g[tcur + 1][cc][lane] += s0[cc];
-                g[tcur + 2][cc][lane] += s1[cc];
-                s0[cc] = s1[cc] = 0.f;
-              }
-            }
+            if (tcur >= -1) flush_run(g, lane, tcur, S, T);
             tcur = t0;
           }
           const float yv = ycol[r];
-#pragma unroll
-          for (int cc = 0; cc < kJC; ++cc) {
-            const float val = yv * wk[cc];
-            s0[cc] = fmaf(1.f - fb, val, s0[cc]);
-            s1[cc] = fmaf(fb, val, s1[cc]);
-          }
+          const float yfb = yv * (xb - fl);
+          const float w0 = fmaxf(0.f, 1.f - fabsf(xc));
+          const float w1 = __builtin_amdgcn_fmed3f(0.f, xc, 2.f - xc);
+          S[0] = fmaf(yv, w0, S[0]);
+          S[1] = fmaf(yv, w1, S[1]);
+          T[0] = fmaf(yfb, w0, T[0]);
+          T[1] = fmaf(yfb, w1, T[1]);
         }
-        if (tcur >= -1) {
-#pragma unroll
-          for (int cc = 0; cc < kJC; ++cc) {
-            g[tcur + 1][cc][lane] += s0[cc];
-            g[tcur + 2][cc][lane] += s1[cc];
-          }
-        }
+        if (tcur >= -1) flush_run(g, lane, tcur, S, T);
       }
     }
   }
@@ -502,7 +504,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
     for (int t = 0; t < kJB; ++t) {
       const int32_t j = jw + t;
       if (j >= A.nb) break;
-      const float v = g[t + 1][cc][lane];
+      const float v = g[t][cc][lane];
       const int64_t o = i * A.sa + j * A.sb + kc * A.sc;
       if (A.mode == 1)
         A.vol[o] += v;
@@ -578,8 +580,16 @@ int kk::fp_transpose_launch(const kk_plan_s* plan, const float* proj,
     A.n_angles = n;
     A.mode = (first && !additive) ? 0 : 1;
     first = false;
-    hipLaunchKernelGGL(cone_fp_t_kernel, dim3(unsigned(blocks)),
-                       dim3(kThreads), 0, st, plan->fp, A);
+    const dim3 grid(static_cast<unsigned>(blocks));
+    if (plan->fp_all_colc)
+      hipLaunchKernelGGL(cone_fp_t_kernel<kPathColumn>, grid,
+                         dim3(kThreads), 0, st, plan->fp, A);
+    else if (!plan->fp_any_colc)
+      hipLaunchKernelGGL(cone_fp_t_kernel<kPathGeneral>, grid,
+                         dim3(kThreads), 0, st, plan->fp, A);
+    else
+      hipLaunchKernelGGL((cone_fp_t_kernel<kPathColumn | kPathGeneral>),
+                         grid, dim3(kThreads), 0, st, plan->fp, A);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_t_kernel");
   }

@@ -183,6 +183,7 @@ struct kk_plan_s {
   int32_t fp_mask = 0;  // bit a: frame texture of ray axis a is used
   bool fp_set = false;
   bool fp_all_colc = false;  // every angle column-coherent (FpAngle.colc)
+  bool fp_any_colc = false;  // some angle column-coherent
   // angle ids grouped by ray axis: fr_idx[fr_start[a] .. fr_start[a+1])
   int32_t* fr_idx = nullptr;
   int32_t fr_cap = 0;

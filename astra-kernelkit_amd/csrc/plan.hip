@@ -244,8 +244,11 @@ extern "C" int kk_set_params_fp(kk_plan* plan, const double* params,
   if (rc != KK_OK) return rc;
   for (int a = 0; a < 4; ++a) plan->fr_start[a] = start[a];
   plan->fp_all_colc = true;
-  for (int32_t k = 0; k < n; ++k)
+  plan->fp_any_colc = false;
+  for (int32_t k = 0; k < n; ++k) {
     plan->fp_all_colc = plan->fp_all_colc && tab[k].colc != 0;
+    plan->fp_any_colc = plan->fp_any_colc || tab[k].colc != 0;
+  }
   plan->fp_n = n;
   for (int t = 0; t < 3; ++t) plan->fp_shape[t] = shape[t];
   plan->fp_mask = mask;

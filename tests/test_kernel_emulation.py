@@ -365,27 +365,30 @@ def _fpt_taps(P, i, rows, cols, nb, nc):
     return [np.concatenate(x) for x in zip(*out)]
 
 
-def _circ_window(ia, betaf, kl, rows, pad=16):
+def _circ_window(ia, betaf, kl, rows, pad=16, with_fast=False):
     """Circular path: the rows cone_fp_t_kernel reads for a lane (first c
     voxel kl) and a column (float32 1 / alpha and beta): nrow rows from the
     ceiling of the window's first row when the window is short, else the
-    exact float32 range widened by 1e-2 row."""
+    exact float32 range widened by 1e-2 row.  The first row is
+    kl / alpha + origin, origin = fmaf(-beta, 1 / alpha, off)."""
     ia = np.asarray(ia, f32)
     with np.errstate(divide="ignore", over="ignore", invalid="ignore"):
         ln = (f32(_KJC + 1) * np.abs(ia)).astype(f32)
         fast = (ia != 0) & (ln + f32(3) <= f32(pad))
         nrow = np.floor(ln + f32(2e-3)).astype(np.int64) + 1
         off = (np.minimum(f32(0), f32(_KJC + 1) * ia) - f32(1e-3)).astype(f32)
-        r1f = _f32fma((kl.astype(f32) - betaf).astype(f32), ia, off)
-        rf = np.ceil(np.where(fast, r1f, 0)).astype(np.int64)
+        org = _f32fma(-np.asarray(betaf, f32), ia, np.where(fast, off,
+                                                               f32(0)))
+        r1 = _f32fma(kl.astype(f32), ia, org)
+        rf = np.ceil(np.where(fast, r1, 0)).astype(np.int64)
         rf = np.minimum(np.maximum(rf, -pad), rows + pad - nrow)
-        r1 = ((kl.astype(f32) - betaf) * ia).astype(f32)
         r2 = _f32fma(_KJC + 1.0, ia, r1)
     r_lo, r_hi = _int_range(np.minimum(r1, r2) - f32(1e-2),
                             np.maximum(r1, r2) + f32(1e-2), rows)
     steep = ia != 0
     r_lo, r_hi = np.where(steep, r_lo, 0), np.where(steep, r_hi, rows - 1)
-    return (np.where(fast, rf, r_lo), np.where(fast, rf + nrow - 1, r_hi))
+    out = (np.where(fast, rf, r_lo), np.where(fast, rf + nrow - 1, r_hi))
+    return out + (fast,) if with_fast else out
 
 
 def _fpt_covered(P, i, R, C, J, K, rows, cols, eps=_KEPS):
@@ -486,10 +489,23 @@ def _fpt_kernel_weights(P, i, R, C, J, K, rows, cols, nb):
         alphaf, betaf = alpha.astype(f32), beta.astype(f32)
         with np.errstate(divide="ignore", over="ignore"):
             ia = np.where(np.abs(alphaf) > f32(1e-9), f32(1) / alphaf, f32(0))
-        r_lo, _ = _circ_window(ia, betaf, kl, rows)
+        r_lo, r_hi, fast = _circ_window(ia, betaf, kl, rows, with_fast=True)
         loc = (alpha * r_lo + beta - (kl + 1.0)).astype(f32)
-        x = _f32fma(alphaf, (R - r_lo).astype(f32), loc)
-        wc = np.maximum(f32(0), f32(1) - np.abs((x - (K - kl)).astype(f32)))
+        # short windows: groups of 4 rows from a float32 anchor
+        # xg = fmaf(alpha, 4 g, loc), x = fmaf(alpha, t, xg) for t > 0, then
+        # the remaining rows at fmaf(alpha, u, loc); long windows
+        # fmaf(alpha, u, loc)
+        u = R - r_lo
+        grouped = fast & (u < ((r_hi - r_lo + 1) // 4) * 4)
+        xg = _f32fma(alphaf, (u - u % 4).astype(f32), loc)
+        x = np.where(grouped & (u % 4 > 0),
+                     _f32fma(alphaf, (u % 4).astype(f32), xg),
+                     np.where(grouped, xg,
+                              _f32fma(alphaf, u.astype(f32), loc)))
+        w0 = np.maximum(f32(0), (f32(1) - np.abs(x)).astype(f32))
+        w1 = np.clip((f32(2) - x).astype(f32), f32(0), None)
+        w1 = np.clip(np.minimum(x, w1), f32(0), None)  # med3(0, x, 2 - x)
+        wc = np.where(K - kl == 0, w0, w1).astype(f32)
         fb64 = fb
     else:
         c_lo, _, r_lo, _, _ = _lane_boxes(P, i, jw, kl, rows, cols)
@@ -509,16 +525,25 @@ def _fpt_kernel_weights(P, i, R, C, J, K, rows, cols, nb):
         ob = (tb * arel + g["taub"] - (jw + lo)).astype(f32)
         oc = (tc * arel + g["tauc"] - (kl + lo)).astype(f32)
         F = {x: f32(g[x]) for x in ("ua", "ub", "uc", "va", "vb", "vc")}
+        tbf, tcf = tb.astype(f32), tc.astype(f32)
         dcf = (C - wc_lo).astype(f32)
         drf = (R - r_lo).astype(f32)
-        dA = _f32fma(drf, F["va"], (dcf * F["ua"]).astype(f32))
-        dB = _f32fma(drf, F["vb"], (dcf * F["ub"]).astype(f32))
-        dC = _f32fma(drf, F["vc"], (dcf * F["uc"]).astype(f32))
-        rcp = (1.0 / (Da0.astype(f32) + dA).astype(np.float64)).astype(f32)
+        # per column: numerators affine in the row offset, slopes
+        # vb - tb va and vc - tc va; denominator Da0 + dc ua + u va
+        dAc = (dcf * F["ua"]).astype(f32)
+        nb0 = _f32fma(-tbf, dAc, (dcf * F["ub"]).astype(f32))
+        nc0 = _f32fma(-tcf, dAc, (dcf * F["uc"]).astype(f32))
+        den0 = (Da0.astype(f32) + dAc).astype(f32)
+        kvb = _f32fma(-tbf, F["va"], F["vb"])
+        kvc = _f32fma(-tcf, F["va"], F["vc"])
+        den = _f32fma(drf, F["va"], den0)
+        rcp = (1.0 / den.astype(np.float64)).astype(f32)
         sc = (f32(arel) * rcp).astype(f32)
-        xb = _f32fma(_f32fma(-tb.astype(f32), dA, dB), sc, ob)
-        xc = _f32fma(_f32fma(-tc.astype(f32), dA, dC), sc, oc)
-        wc = np.maximum(f32(0), f32(1) - np.abs((xc - (K - kl)).astype(f32)))
+        xb = _f32fma(_f32fma(drf, kvb, nb0), sc, ob)
+        xc = _f32fma(_f32fma(drf, kvc, nc0), sc, oc)
+        w0 = np.maximum(f32(0), (f32(1) - np.abs(xc)).astype(f32))
+        w1 = np.clip(np.minimum(xc, (f32(2) - xc).astype(f32)), f32(0), None)
+        wc = np.where(K - kl == 0, w0, w1).astype(f32)
         fl = np.floor(xb)
         t0 = fl.astype(np.int64)
         fb64 = (xb - fl).astype(f32)
