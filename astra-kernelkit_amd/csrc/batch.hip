@@ -10,8 +10,11 @@
 // kk_sirt_fp_residual / kk_sirt_bp_update), builds device descriptors once
 // for a chosen tile shape, and launches every forward (or backward) problem
 // in one persistent grid that walks the problems' concatenated workgroup
-// tiles.  Launches allocate nothing and can be captured in a HIP graph (a
-// SIRT iteration of the whole batch is two launches).
+// tiles.  Texture builds (SIRT residuals into backprojection textures,
+// updated volumes into forward-projection frames) are batched the same way.
+// Launches allocate nothing and can be captured in a HIP graph (a SIRT
+// iteration of the whole batch is four launches: forward projections,
+// residuals, backprojections with the update, frame rebuilds).
 #include <new>
 #include <vector>
 
@@ -21,8 +24,13 @@ struct kk_batch_s {
   int device = 0;
   std::vector<kk::FpSpec> fp_specs;
   std::vector<kk::BpSpec> bp_specs;
+  // texture builds: residuals into BP textures, volumes into FP frames
+  std::vector<kk::TexJob> res_jobs, vol_jobs;
+  int32_t n_res = 0, n_vol = 0;  // problems added (a volume has 1-3 jobs)
   kk::FpBatch* fp = nullptr;
   kk::BpBatch* bp = nullptr;
+  kk::TexJobs* res = nullptr;
+  kk::TexJobs* vol = nullptr;
 };
 
 namespace {
@@ -30,8 +38,12 @@ namespace {
 void release(kk_batch* b) {
   kk::fp_batch_free(b->fp);
   kk::bp_batch_free(b->bp);
+  kk::tex_jobs_free(b->res);
+  kk::tex_jobs_free(b->vol);
   b->fp = nullptr;
   b->bp = nullptr;
+  b->res = nullptr;
+  b->vol = nullptr;
 }
 
 int check_same_device(const kk_batch* b) {
@@ -67,8 +79,13 @@ extern "C" int kk_batch_destroy(kk_batch* b) {
 
 extern "C" int32_t kk_batch_size(const kk_batch* b, int32_t which) {
   if (b == nullptr) return -1;
-  return static_cast<int32_t>(which == 0 ? b->fp_specs.size()
-                                         : b->bp_specs.size());
+  switch (which) {
+    case 0: return static_cast<int32_t>(b->fp_specs.size());
+    case 1: return static_cast<int32_t>(b->bp_specs.size());
+    case 2: return b->n_res;
+    case 3: return b->n_vol;
+    default: return -1;
+  }
 }
 
 extern "C" int kk_batch_add_fp(kk_batch* b, const kk_plan* plan,
@@ -140,6 +157,61 @@ extern "C" int kk_batch_add_sirt_bp(
   return KK_OK;
 }
 
+extern "C" int kk_batch_add_residual(kk_batch* b, const float* proj,
+                                     const int64_t ps[3], const float* y,
+                                     const int64_t ys[3], const float* R,
+                                     const int64_t rs[3], int32_t n_angles,
+                                     int32_t rows, int32_t cols, float* tex) {
+  if (b == nullptr || proj == nullptr || ps == nullptr || y == nullptr ||
+      ys == nullptr || R == nullptr || rs == nullptr || tex == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_batch_add_residual: null");
+  if (n_angles <= 0 || rows <= 0 || cols <= 0)
+    return kk::fail(KK_ERR_INVALID, "kk_batch_add_residual: bad sizes");
+  kk::TexJob J;
+  J.src = proj;
+  J.s0 = ps[0], J.s1 = ps[1], J.s2 = ps[2];
+  J.y = y;
+  J.y0 = ys[0], J.y1 = ys[1], J.y2 = ys[2];
+  J.R = R;
+  J.r0 = rs[0], J.r1 = rs[1], J.r2 = rs[2];
+  J.tex = reinterpret_cast<float2*>(tex);
+  J.n0 = n_angles, J.n1 = rows, J.n2 = cols;
+  release(b);
+  b->res_jobs.push_back(J);
+  ++b->n_res;
+  return KK_OK;
+}
+
+extern "C" int kk_batch_add_volume_texture(kk_batch* b, const float* vol,
+                                           const int32_t shape[3],
+                                           const int64_t st[3], int32_t mask,
+                                           float* tex) {
+  if (b == nullptr || vol == nullptr || shape == nullptr || st == nullptr ||
+      tex == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_batch_add_volume_texture: null");
+  if (shape[0] <= 0 || shape[1] <= 0 || shape[2] <= 0 || mask <= 0 ||
+      mask > 7)
+    return kk::fail(KK_ERR_INVALID, "kk_batch_add_volume_texture: bad args");
+  // one job per frame, at the offsets of kk_volume_to_fp_texture
+  std::vector<kk::TexJob> jobs;
+  int64_t off = 0;  // in float2 elements
+  for (int a = 0; a < 3; ++a) {
+    if (!(mask & (1 << a))) continue;
+    const int fb = kk::kFrameB[a], fc = kk::kFrameC[a];
+    kk::TexJob J;
+    J.src = vol;
+    J.s0 = st[a], J.s1 = st[fb], J.s2 = st[fc];
+    J.tex = reinterpret_cast<float2*>(tex) + off;
+    J.n0 = shape[a], J.n1 = shape[fb], J.n2 = shape[fc];
+    jobs.push_back(J);
+    off += kk::frame_elems(shape, a);
+  }
+  release(b);
+  b->vol_jobs.insert(b->vol_jobs.end(), jobs.begin(), jobs.end());
+  ++b->n_vol;
+  return KK_OK;
+}
+
 extern "C" int kk_batch_configure(kk_batch* b, int32_t fp_rows_per_thread,
                                   int32_t bp_zrun) {
   if (b == nullptr) return kk::fail(KK_ERR_INVALID, "kk_batch_configure");
@@ -157,7 +229,40 @@ extern "C" int kk_batch_configure(kk_batch* b, int32_t fp_rows_per_thread,
       return rc;
     }
   }
+  if (!b->res_jobs.empty()) {
+    rc = kk::tex_jobs_build(b->res_jobs, &b->res);
+    if (rc != KK_OK) {
+      release(b);
+      return rc;
+    }
+  }
+  if (!b->vol_jobs.empty()) {
+    rc = kk::tex_jobs_build(b->vol_jobs, &b->vol);
+    if (rc != KK_OK) {
+      release(b);
+      return rc;
+    }
+  }
   return KK_OK;
+}
+
+extern "C" int kk_batch_residual(const kk_batch* b, void* stream) {
+  if (b == nullptr) return kk::fail(KK_ERR_INVALID, "kk_batch_residual: null");
+  if (b->res_jobs.empty()) return KK_OK;
+  if (b->res == nullptr)
+    return kk::fail(KK_ERR_NOT_SET,
+                    "kk_batch_residual: call kk_batch_configure");
+  return kk::tex_jobs_launch(b->res, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int kk_batch_volume_textures(const kk_batch* b, void* stream) {
+  if (b == nullptr)
+    return kk::fail(KK_ERR_INVALID, "kk_batch_volume_textures: null");
+  if (b->vol_jobs.empty()) return KK_OK;
+  if (b->vol == nullptr)
+    return kk::fail(KK_ERR_NOT_SET,
+                    "kk_batch_volume_textures: call kk_batch_configure");
+  return kk::tex_jobs_launch(b->vol, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int kk_batch_fp(const kk_batch* b, int32_t grid_blocks,

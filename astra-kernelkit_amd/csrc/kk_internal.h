@@ -139,6 +139,26 @@ int bp_batch_build(const std::vector<BpSpec>& specs, int32_t zrun,
 void bp_batch_free(BpBatch* b);
 int bp_batch_launch(const BpBatch* b, int32_t grid_blocks, hipStream_t s);
 
+// One texture build of a batch: the zero-halo float2 texture
+// [n0][n1 + kHalo][n2 + kHalo] of a 3-D array addressed by element strides
+// (an FP frame of a volume, or a BP texture of projections), optionally of
+// the SIRT residual (src - y) * R.
+struct TexJob {
+  const float* src = nullptr;
+  int64_t s0 = 0, s1 = 0, s2 = 0;
+  const float* y = nullptr;  // nullptr: a plain copy
+  int64_t y0 = 0, y1 = 0, y2 = 0;
+  const float* R = nullptr;
+  int64_t r0 = 0, r1 = 0, r2 = 0;
+  float2* tex = nullptr;
+  int32_t n0 = 0, n1 = 0, n2 = 0;
+  int64_t first = 0;  // first texture row of this job in the batch
+};
+struct TexJobs;
+int tex_jobs_build(std::vector<TexJob> jobs, TexJobs** out);
+void tex_jobs_free(TexJobs* j);
+int tex_jobs_launch(const TexJobs* j, hipStream_t s);
+
 // Fill a BpSpec from the C-ABI arguments of kk_cone_bp / kk_sirt_bp_update
 // (validated when the launch arguments are built).
 int bp_spec(BpSpec& S, const kk_plan_s* plan, const float* tex,

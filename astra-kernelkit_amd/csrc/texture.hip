@@ -8,6 +8,9 @@
 // (value, neighbour), so that the four taps of a bilinear sample are one
 // 16-byte load and coordinates are clamped into the halo instead of being
 // bounds-tested per tap.
+#include <new>
+#include <vector>
+
 #include "kk_internal.h"
 
 namespace {
@@ -70,6 +73,56 @@ __global__ __launch_bounds__(256) void bp_tex_kernel(
     if (r1 >= 0 && r1 < rows) v1 = value(r1);
   }
   tex[(int64_t(a) * tr + jv) * tc + ju] = make_float2(v0, v1);
+}
+
+// Many texture builds in one launch: block b converts the texture rows
+// [b chunk, (b + 1) chunk) of the jobs' concatenated rows, finding its
+// first job by binary search over their first rows (wave-uniform) and
+// stepping to the next job when a row passes its end.  Each element is the
+// same arithmetic as fp_frame_kernel / bp_tex_kernel.
+__global__ __launch_bounds__(256) void tex_jobs_kernel(
+    const kk::TexJob* __restrict__ jobs, int32_t n_jobs, int64_t total,
+    int64_t chunk) {
+  const int64_t t0 = int64_t(blockIdx.x) * chunk;
+  if (t0 >= total) return;
+  const int64_t t1 = t0 + chunk < total ? t0 + chunk : total;
+  int32_t lo = 0, hi = n_jobs - 1;
+  while (lo < hi) {
+    const int32_t mid = (lo + hi + 1) >> 1;
+    if (jobs[mid].first <= t0)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  int32_t j = lo;
+  for (int64_t t = t0; t < t1; ++t) {
+    while (j + 1 < n_jobs && jobs[j + 1].first <= t) ++j;
+    const kk::TexJob& J = jobs[j];
+    const int32_t tr = J.n1 + kk::kHalo, tc = J.n2 + kk::kHalo;
+    const int64_t lr = t - J.first;  // texture row within the job
+    const int32_t i = static_cast<int32_t>(lr / tr);
+    const int32_t r0 = static_cast<int32_t>(lr - int64_t(i) * tr) -
+                       kk::kHaloLo;
+    const int32_t r1 = r0 + 1;
+    const bool ok0 = r0 >= 0 && r0 < J.n1, ok1 = r1 >= 0 && r1 < J.n1;
+    auto value = [&](int32_t r, int32_t c) {
+      float v = J.src[i * J.s0 + r * J.s1 + c * J.s2];
+      if (J.y != nullptr)
+        v = (v - J.y[i * J.y0 + r * J.y1 + c * J.y2]) *
+            J.R[i * J.r0 + r * J.r1 + c * J.r2];
+      return v;
+    };
+    float2* out = J.tex + lr * tc;
+    for (int32_t ju = threadIdx.x; ju < tc; ju += 256) {
+      const int32_t c = ju - kk::kHaloLo;
+      float v0 = 0.f, v1 = 0.f;
+      if (c >= 0 && c < J.n2) {
+        if (ok0) v0 = value(r0, c);
+        if (ok1) v1 = value(r1, c);
+      }
+      out[ju] = make_float2(v0, v1);
+    }
+  }
 }
 
 __global__ __launch_bounds__(256) void fill_kernel(float* __restrict__ p,
@@ -167,3 +220,62 @@ extern "C" int kk_fill(float* ptr, int64_t n, float value, void* stream) {
   if (e != hipSuccess) return kk::hip_fail(e, "fill_kernel");
   return KK_OK;
 }
+
+namespace kk {
+
+struct TexJobs {
+  TexJob* jobs = nullptr;
+  int32_t n = 0;
+  int64_t total = 0;  // texture rows of all jobs
+};
+
+int tex_jobs_build(std::vector<TexJob> jobs, TexJobs** out) {
+  if (jobs.empty()) return fail(KK_ERR_INVALID, "empty texture batch");
+  int64_t total = 0;
+  for (TexJob& J : jobs) {
+    if (J.src == nullptr || J.tex == nullptr || J.n0 <= 0 || J.n1 <= 0 ||
+        J.n2 <= 0)
+      return fail(KK_ERR_INVALID, "texture batch: bad job");
+    if (int64_t(J.n0) * (J.n1 + kHalo) >= (int64_t(1) << 31))
+      return fail(KK_ERR_UNSUPPORTED, "texture batch: job too large");
+    J.first = total;
+    total += int64_t(J.n0) * (J.n1 + kHalo);
+  }
+  auto* b = new (std::nothrow) TexJobs();
+  if (b == nullptr) return fail(KK_ERR_INVALID, "out of host memory");
+  b->n = static_cast<int32_t>(jobs.size());
+  b->total = total;
+  hipError_t e = hipMalloc(&b->jobs, sizeof(TexJob) * jobs.size());
+  if (e == hipSuccess)
+    e = hipMemcpy(b->jobs, jobs.data(), sizeof(TexJob) * jobs.size(),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    tex_jobs_free(b);
+    return hip_fail(e, "texture batch descriptors");
+  }
+  *out = b;
+  return KK_OK;
+}
+
+void tex_jobs_free(TexJobs* b) {
+  if (b == nullptr) return;
+  if (b->jobs) hipFree(b->jobs);
+  delete b;
+}
+
+int tex_jobs_launch(const TexJobs* b, hipStream_t stream) {
+  if (b == nullptr) return fail(KK_ERR_NOT_SET, "texture batch not built");
+  // ~16 rows of a few hundred float2 per workgroup: enough workgroups to
+  // fill the device, few enough to amortise each one's job search
+  int64_t blocks = (b->total + 15) / 16;
+  if (blocks > 65536) blocks = 65536;
+  const int64_t chunk = (b->total + blocks - 1) / blocks;
+  blocks = (b->total + chunk - 1) / chunk;
+  hipLaunchKernelGGL(tex_jobs_kernel, dim3(static_cast<unsigned>(blocks)),
+                     dim3(256), 0, stream, b->jobs, b->n, b->total, chunk);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, "tex_jobs_kernel");
+  return KK_OK;
+}
+
+}  // namespace kk

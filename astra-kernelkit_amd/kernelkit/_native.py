@@ -108,6 +108,13 @@ _SIGNATURES = {
                                           ctypes.c_int32]),
     "kk_batch_fp": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
     "kk_batch_bp": (ctypes.c_int, [_vp, ctypes.c_int32, _vp]),
+    "kk_batch_add_residual": (ctypes.c_int, [
+        _vp, _vp, _c_i64p, _vp, _c_i64p, _vp, _c_i64p, ctypes.c_int32,
+        ctypes.c_int32, ctypes.c_int32, _vp]),
+    "kk_batch_add_volume_texture": (ctypes.c_int, [
+        _vp, _vp, _c_i32p, _c_i64p, ctypes.c_int32, _vp]),
+    "kk_batch_residual": (ctypes.c_int, [_vp, _vp]),
+    "kk_batch_volume_textures": (ctypes.c_int, [_vp, _vp]),
     "kk_probe_l2_read": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32,
                                         ctypes.c_int32, _vp, _vp]),
     "kk_probe_gather": (ctypes.c_int, [

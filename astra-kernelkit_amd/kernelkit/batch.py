@@ -223,6 +223,15 @@ class BatchSirt:
                 0 if max_constraint is None else 1,
                 None, 0,   # frames of x: rebuilt after the update
                 float(vg.voxel_volume())))
+            # the texture builds between the two operators, batched too
+            _native.check(lib.kk_batch_add_residual(
+                b.ptr, _native._vp(p.data_ptr()), _native.i64(p.stride()),
+                _native._vp(y.data_ptr()), _native.i64(y.stride()),
+                _native._vp(R.data_ptr()), _native.i64(R.stride()), n, rows,
+                cols, _native._vp(rtex.ptr)))
+            _native.check(lib.kk_batch_add_volume_texture(
+                b.ptr, _native._vp(x.data_ptr()), _native.i32(shape),
+                _native.i64(x.stride()), xtex.mask, _native._vp(xtex.ptr)))
             self.x.append(x)
             self.C.append(C)
             self.R.append(R)
@@ -248,26 +257,21 @@ class BatchSirt:
         self.reset()
 
     def iterate(self):
-        """One SIRT iteration of every problem: one persistent launch per
-        operator with the residual pass of each problem between them, then
-        the refresh of each x's forward-projection frames (coalesced
-        texture kernels; the update epilogue's scattered frame stores cost
-        several times more)."""
+        """One SIRT iteration of every problem in four launches whatever
+        the number of problems: all forward projections, all residual
+        textures ``R (A x - y)``, all backprojections with the update
+        epilogue, all rebuilds of the x's forward-projection frames
+        (coalesced texture kernels; the update epilogue's scattered frame
+        stores cost several times more)."""
         if self.cfg_fp is None:
             self.configure(tune=False)
         lib = _native.lib()
         st = _native.stream_ptr()
-        _native.check(lib.kk_batch_fp(self.handle.ptr, self.cfg_fp.grid, st))
-        for p, y, R, t in zip(self.p, self.y, self.R, self.res_tex):
-            n, rows, cols = p.shape
-            _native.check(lib.kk_residual_to_bp_texture(
-                _native._vp(p.data_ptr()), _native.i64(p.stride()),
-                _native._vp(y.data_ptr()), _native.i64(y.stride()),
-                _native._vp(R.data_ptr()), _native.i64(R.stride()),
-                n, rows, cols, _native._vp(t.ptr), st))
-        _native.check(lib.kk_batch_bp(self.handle.ptr, self.cfg_bp.grid, st))
-        for t, x in zip(self._xtex, self.x):
-            t.update(x)
+        h = self.handle.ptr
+        _native.check(lib.kk_batch_fp(h, self.cfg_fp.grid, st))
+        _native.check(lib.kk_batch_residual(h, st))
+        _native.check(lib.kk_batch_bp(h, self.cfg_bp.grid, st))
+        _native.check(lib.kk_batch_volume_textures(h, st))
 
     def capture(self, iters: int):
         """Capture ``iters`` iterations in a HIP graph (torch.cuda.graph);

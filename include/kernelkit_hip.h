@@ -223,12 +223,13 @@ int64_t kk_cone_fp_transpose_static_capacity(void);
  * kk_batch_fp / kk_batch_bp run all forward / backward problems in one
  * persistent launch of grid_blocks workgroups (0 = one per tile).  Launches
  * allocate nothing and are graph-capturable; a SIRT iteration of the batch
- * is kk_batch_fp then kk_batch_bp.  Adding a problem drops the descriptors
- * (configure again). */
+ * is kk_batch_fp, kk_batch_residual, kk_batch_bp, kk_batch_volume_textures.
+ * Adding a problem drops the descriptors (configure again). */
 typedef struct kk_batch_s kk_batch;
 int kk_batch_create(kk_batch** out);
 int kk_batch_destroy(kk_batch* batch);
-/* Number of forward (which = 0) or backward (which = 1) problems. */
+/* Number of forward (which = 0) or backward (which = 1) problems, of
+ * residual texture builds (2) or of volume texture builds (3). */
 int32_t kk_batch_size(const kk_batch* batch, int32_t which);
 int kk_batch_add_fp(kk_batch* batch, const kk_plan* plan,
                     const float* vol_tex, float* proj,
@@ -252,10 +253,25 @@ int kk_batch_add_sirt_bp(kk_batch* batch, const kk_plan* plan,
                          const int64_t mask_strides[3], float lo, float hi,
                          int32_t clip_lo, int32_t clip_hi, float* x_tex,
                          int32_t x_tex_mask, float output_scale);
+/* Texture builds of a batch, each kind in one launch: the SIRT residual
+ * texture of kk_residual_to_bp_texture ((proj - y) * R into tex), and the
+ * forward-projection frames of kk_volume_to_fp_texture (vol into tex for
+ * the frames in mask); same arguments, same results bit for bit. */
+int kk_batch_add_residual(kk_batch* batch, const float* proj,
+                          const int64_t proj_strides[3], const float* y,
+                          const int64_t y_strides[3], const float* R,
+                          const int64_t R_strides[3], int32_t n_angles,
+                          int32_t rows, int32_t cols, float* tex);
+int kk_batch_add_volume_texture(kk_batch* batch, const float* vol,
+                                const int32_t vol_shape[3],
+                                const int64_t vol_strides[3], int32_t mask,
+                                float* tex);
 int kk_batch_configure(kk_batch* batch, int32_t fp_rows_per_thread,
                        int32_t bp_zrun);
 int kk_batch_fp(const kk_batch* batch, int32_t grid_blocks, void* stream);
 int kk_batch_bp(const kk_batch* batch, int32_t grid_blocks, void* stream);
+int kk_batch_residual(const kk_batch* batch, void* stream);
+int kk_batch_volume_textures(const kk_batch* batch, void* stream);
 
 /* ---- host-only introspection (no device calls) ---------------------- */
 

@@ -147,3 +147,65 @@ def test_batch_shape_errors(device):
         B.fp([_dev(np.zeros((3, 3, 3)))] * 2)
     with pytest.raises(ValueError):
         B.fp([_dev(np.zeros(P[0][1].shape))])
+
+
+def test_batch_texture_builds_match_single(device):
+    """kk_batch_residual / kk_batch_volume_textures write, bit for bit, the
+    textures of the per-problem kk_residual_to_bp_texture /
+    kk_volume_to_fp_texture calls, over jobs of different shapes, strided
+    inputs and frame masks."""
+    torch = _torch()
+    from kernelkit import _native
+    lib = _native.lib()
+    st = _native.stream_ptr()
+    g = torch.Generator(device="cuda").manual_seed(3)
+    b = _native.Batch()
+    res_cases, vol_cases = [], []
+    keep = []      # the batch holds raw pointers: keep its inputs alive
+    for n, rows, cols in ((3, 7, 11), (5, 17, 300), (2, 1, 1)):
+        base = torch.rand((n, rows, cols + 2), device="cuda", generator=g)
+        p = base[:, :, 1:cols + 1]          # row-strided
+        y = torch.rand((n, rows, cols), device="cuda", generator=g)
+        R = torch.rand((cols, rows, n), device="cuda",
+                       generator=g).permute(2, 1, 0)   # transposed strides
+        size = n * (rows + 3) * (cols + 3) * 2
+        t_batch = torch.full((size,), 7.0, device="cuda")
+        t_single = torch.full((size,), 7.0, device="cuda")
+        _native.check(lib.kk_batch_add_residual(
+            b.ptr, _native._vp(p.data_ptr()), _native.i64(p.stride()),
+            _native._vp(y.data_ptr()), _native.i64(y.stride()),
+            _native._vp(R.data_ptr()), _native.i64(R.stride()), n, rows,
+            cols, _native._vp(t_batch.data_ptr())))
+        _native.check(lib.kk_residual_to_bp_texture(
+            _native._vp(p.data_ptr()), _native.i64(p.stride()),
+            _native._vp(y.data_ptr()), _native.i64(y.stride()),
+            _native._vp(R.data_ptr()), _native.i64(R.stride()), n, rows,
+            cols, _native._vp(t_single.data_ptr()), st))
+        res_cases.append((t_batch, t_single))
+        keep += [base, y, R]
+    for shape, mask in (((9, 5, 6), 7), ((4, 33, 2), 3), ((1, 1, 70), 4)):
+        v = torch.rand(tuple(reversed(shape)), device="cuda",
+                       generator=g).permute(2, 1, 0)    # world strides
+        elems = lib.kk_fp_texture_elems(_native.i32(shape), mask)
+        t_batch = torch.full((elems,), 7.0, device="cuda")   # floats
+        t_single = torch.full((elems,), 7.0, device="cuda")
+        _native.check(lib.kk_batch_add_volume_texture(
+            b.ptr, _native._vp(v.data_ptr()), _native.i32(shape),
+            _native.i64(v.stride()), mask, _native._vp(t_batch.data_ptr())))
+        _native.check(lib.kk_volume_to_fp_texture(
+            _native._vp(v.data_ptr()), _native.i32(shape),
+            _native.i64(v.stride()), mask, _native._vp(t_single.data_ptr()),
+            st))
+        vol_cases.append((t_batch, t_single))
+        keep.append(v)
+    assert lib.kk_batch_size(b.ptr, 2) == 3
+    assert lib.kk_batch_size(b.ptr, 3) == 3
+    # not configured yet
+    with pytest.raises(Exception):
+        _native.check(lib.kk_batch_residual(b.ptr, st))
+    _native.check(lib.kk_batch_configure(b.ptr, 0, 8))
+    _native.check(lib.kk_batch_residual(b.ptr, st))
+    _native.check(lib.kk_batch_volume_textures(b.ptr, st))
+    torch.cuda.synchronize()
+    for t_batch, t_single in res_cases + vol_cases:
+        assert torch.equal(t_batch, t_single)
