@@ -392,7 +392,7 @@ def empty(shape, like=None, dtype="float32"):
         torch = _torch_module()
         return torch.empty(shape, dtype=getattr(torch, dtype),
                            device=like.device)
-    dev = None if like is None else asarray(like).device.index
+    dev = asarray(like).device.index if _is_device(like) else None
     return DeviceArray(shape, np.dtype(dtype), device=dev)
 
 

@@ -25,6 +25,25 @@ def _zeros(shape, like=None):
     return device.zeros(shape, like=like)
 
 
+def _operands(input, out, domain_shape, range_shape):
+    """``(x, y, out)``: the input and output as arrays the projectors take
+    (``__cuda_array_interface__`` arrays through
+    :class:`~kernelkit.DeviceArray` views, host input copied to the
+    device), and the output to return (allocated, of the input's kind, if
+    not given)."""
+    from kernelkit import device
+    from kernelkit.kernel import as_device_array, is_device_array
+    x = as_device_array(input)
+    if tuple(x.shape) != tuple(domain_shape):
+        raise ValueError(
+            f"Input shape {tuple(x.shape)} does not match domain shape "
+            f"{tuple(domain_shape)} of operator.")
+    if out is None:
+        out = _zeros(range_shape, x)
+        return x, out, out
+    return x, (device.asarray(out) if is_device_array(out) else out), out
+
+
 class BaseOperator(ABC):
     """A linear operator with domain/range shapes and an adjoint."""
 
@@ -67,14 +86,10 @@ class ProjectorOperator(BaseOperator, ABC):
     def __call__(self, input, out=None, additive=False):
         """``out (+)= A(input)``; allocates a zero output if none is
         given."""
-        if tuple(input.shape) != self.domain_shape:
-            raise ValueError(
-                f"Input shape {tuple(input.shape)} does not match domain "
-                f"shape {self.domain_shape} of operator.")
-        if out is None:
-            out = _zeros(self.range_shape, input)
-        self.projector.volume = input
-        self.projector.projections = out
+        x, y, out = _operands(input, out, self.domain_shape,
+                              self.range_shape)
+        self.projector.volume = x
+        self.projector.projections = y
         self.projector(additive=additive)
         return out
 
@@ -98,14 +113,10 @@ class ProjectorOperator(BaseOperator, ABC):
                 return outer.domain_shape
 
             def __call__(self, input, out=None, additive=False):
-                if tuple(input.shape) != self.domain_shape:
-                    raise ValueError(
-                        f"Input shape {tuple(input.shape)} does not match "
-                        f"domain shape {self.domain_shape} of operator.")
-                if out is None:
-                    out = _zeros(self.range_shape, input)
-                self.projector.projections = input
-                self.projector.volume = out
+                x, y, out = _operands(input, out, self.domain_shape,
+                                      self.range_shape)
+                self.projector.projections = x
+                self.projector.volume = y
                 self.projector(additive)
                 return out
 
@@ -135,16 +146,11 @@ class ProjectorOperator(BaseOperator, ABC):
                 return outer.domain_shape
 
             def __call__(self, input, out=None, additive=False):
-                if tuple(input.shape) != self.domain_shape:
-                    raise ValueError(
-                        f"Input shape {tuple(input.shape)} does not match "
-                        f"domain shape {self.domain_shape} of operator.")
-                if out is None:
-                    out = _zeros(self.range_shape, input)
+                x, y, out = _operands(input, out, self.domain_shape,
+                                      self.range_shape)
                 p = outer.projector
                 p._require_geometry()
-                p.kernel.transpose(input, p._proj_geom_seq,
-                                   p.volume_geometry, out,
+                p.kernel.transpose(x, p._proj_geom_seq, p.volume_geometry, y,
                                    additive=additive)
                 return out
 
