@@ -15,7 +15,10 @@ The JSON line also carries:
 * ``roofline`` for the dominant kernel (the forward projection): achieved
   gather bytes per second = 16 B per bilinear sample x samples per launch /
   average kernel time (HIP events on the launching stream), against the
-  aggregate L2 read rate 8 XCD x 16 channels x 128 B/clk x 2.4 GHz;
+  gather ceiling: the vector-memory return path of 64 B/clk per CU x 256
+  CUs x 2.4 GHz, next to this run's probe rates of 16-byte gathers in the
+  projectors' access shape (``kk_probe_gather``) from L1-, L2- and
+  HBM-resident footprints;
   ``traffic`` is the HBM bytes per launch from the rocprofv3 FETCH_SIZE +
   WRITE_SIZE counters (scaled by a plain-copy calibration), read from
   ``profiles/cone_fp_bp_512cube_720_traffic.json`` when present, and
@@ -42,6 +45,9 @@ for _p in (ROOT, os.path.join(ROOT, "astra-kernelkit_amd")):
 METRIC = ("cone FP/BP GUPS at 512³/720 proj; batched + autograd vs "
           "plain, same GPU")
 L2_PEAK_GBS = 8 * 16 * 128 * 2.4  # GB/s: 8 XCD x 16 ch x 128 B/clk x 2.4 GHz
+# GB/s: vector-memory return path of 64 B/clk per CU x 256 CUs x 2.4 GHz,
+# the rate at which 16-byte gathers reach the registers even from L1
+L1_RETURN_GBS = 256 * 64 * 2.4
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles",
                             "cone_fp_bp_512cube_720_traffic.json")
@@ -292,23 +298,31 @@ def _small_run(pool, vol, y, sirt_iters):
     from oracle.algorithms import _safe_reciprocal
 
     t = {}
+    tag = f"cpu {pool.workers} core(s)"
     t0 = time.perf_counter()
     pool.fp(vol)
     t["fp"] = time.perf_counter() - t0
+    _log(f"{tag}: FP {t['fp']:.1f} s")
     t0 = time.perf_counter()
     pool.bp(y)
     t["bp"] = time.perf_counter() - t0
+    _log(f"{tag}: BP {t['bp']:.1f} s")
     t0 = time.perf_counter()
     pool.bp(oracle.filter_rows(oracle.preweight(y, pool.pgs), "ramlak"))
     t["fdk"] = time.perf_counter() - t0
+    _log(f"{tag}: FDK {t['fdk']:.1f} s")
     x = None
     if sirt_iters:
         t0 = time.perf_counter()
         x = np.zeros(vol.shape)
         C = _safe_reciprocal(pool.bp(np.ones_like(y)))
+        _log(f"{tag}: SIRT C at {time.perf_counter() - t0:.1f} s")
         R = _safe_reciprocal(pool.fp(np.ones(vol.shape)))
-        for _ in range(sirt_iters):
+        _log(f"{tag}: SIRT R at {time.perf_counter() - t0:.1f} s")
+        for k in range(sirt_iters):
             x = x - C * pool.bp((pool.fp(x) - y) * R)
+            _log(f"{tag}: SIRT iteration {k + 1} at "
+                 f"{time.perf_counter() - t0:.1f} s")
         t["sirt"] = time.perf_counter() - t0
     return t, x
 
@@ -422,11 +436,44 @@ def _event_ms(fn, stream, reps=1):
     return a.elapsed_time(b) / reps
 
 
-def batched_sirt(n_problems=64, iters=30, seed=0, scale=1.0):
-    """SIRT on 64 uneven problems: the whole batch per iteration in two
-    persistent launches, the loop captured in a HIP graph, against the
+def _batch_kernel_ms(st, stream, reps=3):
+    """Milliseconds of each of a batched SIRT iteration's four launches
+    (HIP events around each, eager), averaged over ``reps`` iterations;
+    the state is reset afterwards."""
+    import torch
+
+    from kernelkit import _native
+    lib = _native.lib()
+    h = st.handle.ptr
+    sp = _native.stream_ptr(stream)
+    steps = {
+        "fp": lambda: lib.kk_batch_fp(h, st.cfg_fp.grid, sp),
+        "residual": lambda: lib.kk_batch_residual(h, sp),
+        "bp": lambda: lib.kk_batch_bp(h, st.cfg_bp.grid, sp),
+        "volume_textures": lambda: lib.kk_batch_volume_textures(h, sp),
+    }
+    ms = {k: 0.0 for k in steps}
+    for _ in range(reps):
+        for k, fn in steps.items():
+            a = torch.cuda.Event(enable_timing=True)
+            b = torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            _native.check(fn())
+            b.record(stream)
+            b.synchronize()
+            ms[k] += a.elapsed_time(b) / reps
+    st.reset()
+    return {k: round(v, 3) for k, v in ms.items()}
+
+
+def batched_sirt(n_problems=64, iters=30, seed=0, scale=1.0, classes=True):
+    """SIRT on 64 uneven problems: the whole batch per iteration in four
+    launches (forward projections, residuals, backprojections with the
+    update, frame rebuilds), the loop captured in a HIP graph, against the
     per-problem path (``algo.sirt``'s per-iteration launches, problem after
-    problem).  Setup (textures, weights, tuning, capture) is not timed."""
+    problem).  Setup (textures, weights, tuning, capture) is not timed.
+    Also: the four launches' own times (the graph's overhead over them),
+    and with ``classes`` the same comparison per volume-size class."""
     import numpy as np
     import torch
 
@@ -450,6 +497,7 @@ def batched_sirt(n_problems=64, iters=30, seed=0, scale=1.0):
     t_graph = _event_ms(lambda: st._graph[1].replay(), stream)
     st.reset()
     t_eager = _event_ms(lambda: [st.iterate() for _ in range(iters)], stream)
+    kernel_ms = _batch_kernel_ms(st, stream)
     plain = [algo._SirtState(y, pg, vg, None, None, None)
              for (pg, vg), y in zip(problems, ys)]
     torch.cuda.synchronize()
@@ -459,11 +507,29 @@ def batched_sirt(n_problems=64, iters=30, seed=0, scale=1.0):
             for s in plain:
                 s.iterate()
     t_plain = _event_ms(loop, stream)
+    # the batch's x were reset after the kernel timing: run the iterations
+    # again (graph)
+    st.run(iters, graph=True)
+    torch.cuda.synchronize()
     diff = max(float((a - b).abs().max() / b.abs().max().clamp_min(1e-30))
                for a, b in zip(st.x, [s.x for s in plain]))
     del plain, st
+    per_class = None
+    if classes:
+        per_class = {}
+        bounds = (("64-128", 0, 128), ("129-192", 129, 192),
+                  ("193-256", 193, 256))
+        for name, lo, hi in bounds:
+            idx = [k for k, (_, vg) in enumerate(problems)
+                   if lo * scale <= max(vg.shape) <= hi * scale]
+            if not idx:
+                continue
+            sub = _batch_vs_plain([problems[k] for k in idx],
+                                  [ys[k] for k in idx], iters, stream)
+            per_class[name] = dict(problems=len(idx), **sub)
     lo_v, hi_v = (max(8, int(round(s * scale))) for s in (64, 256))
     lo_d, hi_d = int(96 * scale), int(384 * scale)
+    kernel_sum = sum(kernel_ms.values())
     return {
         "workload": (f"{n_problems} problems, {lo_v}^3-{hi_v}^3 (cubic and "
                      f"anisotropic), 90-360 angles, {lo_d}^2-{hi_d}^2 "
@@ -478,7 +544,43 @@ def batched_sirt(n_problems=64, iters=30, seed=0, scale=1.0):
                               1),
         "max_rel_diff_vs_per_problem": diff,
         "tiles": {"fp": st_cfg_repr(B, "fp"), "bp": st_cfg_repr(B, "bp")},
+        "launches_per_iteration": 4,
+        "kernel_ms_per_iteration": kernel_ms,
+        "graph_ms_per_iteration": round(t_graph / iters, 3),
+        "graph_overhead_vs_kernels": round(
+            t_graph / iters / kernel_sum - 1.0, 4),
+        "per_class": per_class,
     }
+
+
+def _batch_vs_plain(problems, ys, iters, stream):
+    """GUPS of ``iters`` batched SIRT iterations (graph) and of the
+    per-problem loop on a subset of problems."""
+    import numpy as np
+
+    from kernelkit import algo
+    from kernelkit.batch import ConeBatch
+
+    B = ConeBatch(problems)
+    st = B.sirt_state(ys)
+    st.configure(tune=True)
+    st.capture(iters)
+    t_graph = _event_ms(lambda: st._graph[1].replay(), stream)
+    del st
+    plain = [algo._SirtState(y, pg, vg, None, None, None)
+             for (pg, vg), y in zip(problems, ys)]
+
+    def loop():
+        for _ in range(iters):
+            for s in plain:
+                s.iterate()
+    t_plain = _event_ms(loop, stream)
+    del plain
+    upd = 2 * iters * sum(int(np.prod(vg.shape)) * len(pg)
+                          for pg, vg in problems)
+    return {"batched_gups": round(upd / (t_graph * 1e-3) / 1e9, 1),
+            "per_problem_gups": round(upd / (t_plain * 1e-3) / 1e9, 1),
+            "speedup": round(t_plain / t_graph, 3)}
 
 
 def st_cfg_repr(B, which):
@@ -808,7 +910,9 @@ def main(argv=None):
     l2_rate = l2_read_rate(stream)
     _log(f"L2 read probe: {l2_rate:.0f} GB/s")
     gathers = gather_ceilings(stream)
-    ceiling = max(v for k, v in gathers.items() if k.endswith("_l1"))
+    probe_best = max(v for k, v in gathers.items() if k.endswith("_l1"))
+    # the ceiling: the return-path rate, or the probes if they beat it
+    ceiling = round(max(L1_RETURN_GBS, probe_best), 1)
     _log(f"gather probes (GB/s): {gathers}")
     ms_step = wall * 1e3 / args.steps
     updates = int(np.prod(shape)) * n
@@ -851,12 +955,16 @@ def main(argv=None):
             "bound": "gather",
             "kernel": "cone_fp_kernel",
             "achieved": round(achieved, 1),
-            # measured in this run: the best rate of 16-byte gathers in
-            # the projectors' access shape from an L1-resident footprint
+            # the gather ceiling: 64 B/clk/CU x 256 CUs x 2.4 GHz, which
+            # the L1-resident 16-byte gather probes of this run reach
+            # within a few percent ("gather_probes_GBs")
             "peak": ceiling,
-            "peak_source": "kk_probe_gather, best L1-resident shape",
+            "peak_source": "64 B/clk/CU return path; L1-resident "
+                           "kk_probe_gather reaches "
+                           f"{probe_best / L1_RETURN_GBS:.3f} of it",
             "unit": "GB/s",
             "frac": round(achieved / ceiling, 4),
+            "frac_of_best_probe": round(achieved / probe_best, 4),
             "traffic": traffic,
             "algorithmic_bytes": fp_bytes,
             "bytes_per_update": round(fp_bytes / updates, 4),
@@ -895,7 +1003,7 @@ def main(argv=None):
         out["batched"] = batched_sirt()
         # the same generator at a quarter of the sizes: problems small
         # enough for per-problem launches to be latency-bound
-        out["batched_small"] = batched_sirt(scale=0.25)
+        out["batched_small"] = batched_sirt(scale=0.25, classes=False)
     if "autograd" in extras:
         _log("autograd (exact transpose) vs plain ...")
         out["autograd"] = autograd_diff(ceiling)
