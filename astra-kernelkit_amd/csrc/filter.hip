@@ -211,6 +211,11 @@ __global__ __launch_bounds__(256) void filter_rows_kernel(FilterArgs A) {
 
 }  // namespace
 
+// Longest row transform: the signal stays in LDS (8 bytes per point,
+// 128 KiB at 16384 points, within gfx950's 160 KiB per workgroup), so
+// detectors up to 8192 columns filter in one pass.
+constexpr int32_t kMaxFilterN = 16384;
+
 struct kk_filter_s {
   int device = 0;
   int32_t n = 0;
@@ -246,9 +251,10 @@ extern "C" int kk_filter_set_response(kk_filter* f, const double* H,
                                       int32_t n) {
   if (f == nullptr || H == nullptr)
     return kk::fail(KK_ERR_INVALID, "kk_filter_set_response: null");
-  if (n < 2 || (n & (n - 1)) != 0 || n > 8192)
+  if (n < 2 || (n & (n - 1)) != 0 || n > kMaxFilterN)
     return kk::fail(KK_ERR_INVALID,
-                    "filter length must be a power of two in [2, 8192]");
+                    "filter length must be a power of two in [2, %d]",
+                    kMaxFilterN);
   int log2n = 0;
   while ((1 << log2n) < n) ++log2n;
   std::vector<float2> tw(n / 2);
@@ -382,12 +388,33 @@ static int filter_rows(const kk_filter* f, const float* in,
   if (pairs >= (int64_t(1) << 31))
     return kk::fail(KK_ERR_UNSUPPORTED, "kk_filter_rows: too many rows");
   const size_t lds_tw = sizeof(float2) * (size_t(f->n) + size_t(f->n) / 2);
-  if (lds_tw <= 65536)
+  const size_t lds = sizeof(float2) * size_t(f->n);
+  if (lds_tw <= 65536) {
     hipLaunchKernelGGL(filter_rows_kernel<true>, dim3(unsigned(pairs)),
                        dim3(256), lds_tw, stream, A);
-  else
+  } else {
+    if (lds > 65536) {
+      // beyond 64 KiB of dynamic LDS: check the device limit, and raise
+      // the kernel's
+      int dev = 0, limit = 0;
+      hipError_t e = hipGetDevice(&dev);
+      if (e == hipSuccess)
+        e = hipDeviceGetAttribute(
+            &limit, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+      if (e != hipSuccess) return kk::hip_fail(e, "LDS limit");
+      if (size_t(limit) < lds)
+        return kk::fail(KK_ERR_UNSUPPORTED,
+                        "kk_filter_rows: a %d-point row needs %zu bytes of "
+                        "LDS, the device allows %d",
+                        f->n, lds, limit);
+      e = hipFuncSetAttribute(
+          reinterpret_cast<const void*>(&filter_rows_kernel<false>),
+          hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds));
+      if (e != hipSuccess) return kk::hip_fail(e, "hipFuncSetAttribute");
+    }
     hipLaunchKernelGGL(filter_rows_kernel<false>, dim3(unsigned(pairs)),
-                       dim3(256), sizeof(float2) * size_t(f->n), stream, A);
+                       dim3(256), lds, stream, A);
+  }
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return kk::hip_fail(e, "filter_rows_kernel");
   return KK_OK;

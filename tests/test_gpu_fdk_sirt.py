@@ -33,10 +33,11 @@ def _case(n_angles=24, shape=(32, 36, 28), rows=40, cols=48, seed=21, **kw):
 
 @pytest.mark.parametrize("name", ["ramlak", "ramp", "cosine", "hann",
                                   "hamming", "shepp-logan"])
-@pytest.mark.parametrize("cols", [31, 48, 160, 700, 1536, 2500])
+@pytest.mark.parametrize("cols", [31, 48, 160, 700, 1536, 2500, 5000])
 def test_filter_rows(name, cols, device):
-    """Row widths up to 2500 (filter length N = 8192): N <= 4096 keeps the
-    twiddles in LDS, N = 8192 reads them from global memory."""
+    """Row widths up to 5000 (filter length N = 16384): N <= 4096 keeps the
+    twiddles in LDS, N = 8192 and 16384 read them from global memory, and
+    N = 16384 holds 128 KiB of signal in LDS."""
     rng = np.random.default_rng(cols)
     proj = rng.uniform(size=(3, 5, cols))
     ref = oracle.filter_rows(proj, name)
