@@ -195,11 +195,13 @@ class BatchSirt:
             _native.check(lib.kk_recip_or_zero(_native._vp(R.data_ptr()),
                                                R.numel(), st))
             x = torch.zeros(shape, dtype=torch.float32, device=dev)
-            op.projector.volume = x            # FP texture of x (zeros)
-            xtex = op.projector._texture
+            # the state's own FP texture of x: the operator's projector
+            # texture is rebuilt by any later call of the operator
+            fk = op.projector.kernel
+            op.projector._require_geometry()
+            xtex = fk.volume_texture(x, vg)
             rtex = ProjectionTexture(n, rows, cols, device=dev)
             rtex.zero()
-            fk = op.projector.kernel
             bk = op.backprojector.kernel
             # residual in two steps: the batch FP into a buffer, then one
             # coalesced (p - y) R pass per problem into its BP texture (the

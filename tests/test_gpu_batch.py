@@ -139,6 +139,33 @@ def test_batch_sirt_matches_plain_and_graph_replays(device):
                                    atol=2e-5 * float(r.abs().max()))
 
 
+def test_batch_sirt_state_owns_its_textures(device):
+    """Operator calls on the batch between SIRT iterations (which rebuild
+    the operators' own projector textures) leave the SIRT state's forward
+    textures alone: the iterates are bit-equal to an uninterrupted run."""
+    torch = _torch()
+    from kernelkit.batch import ConeBatch
+    P = _problems(3, seed=11)
+    vols = [_dev(v) for v in _volumes(P, seed=2)]
+    B = ConeBatch(P)
+    ys = B.fp(vols, tune=False)
+    ys = [y.clone() for y in ys]
+    st = B.sirt_state(ys)
+    st.configure(tune=False)
+    st.run(4, graph=False)
+    ref = [x.clone() for x in st.x]
+    st.reset()
+    st.run(2, graph=False)
+    other = [torch.rand_like(v) for v in vols]
+    B.fp(other, tune=False)                    # rebuilds projector textures
+    for op, v in zip(B.ops, other):
+        op(v)
+    st.run(2, graph=False)
+    torch.cuda.synchronize()
+    for x, r in zip(st.x, ref):
+        assert torch.equal(x, r)
+
+
 def test_batch_shape_errors(device):
     from kernelkit.batch import ConeBatch
     P = _problems(2)
