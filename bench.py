@@ -327,12 +327,56 @@ def _small_run(pool, vol, y, sirt_iters):
     return t, x
 
 
-def cpu_small(workers=16, sirt_iters=20, single_sirt_iters=1):
+def gpu_small(sirt_iters=20):
+    """The same small-workload jobs on the GPU (inputs resident, HIP events
+    on the current stream): FP, BP, Ram-Lak FDK (``fdk``'s pre-weight and
+    filter fused into the BP texture, then the BP) and ``sirt_iters`` SIRT
+    iterations with their setup, in milliseconds."""
+    import torch
+
+    import kernelkit as kk
+    from kernelkit import workloads
+    from kernelkit.kernel import ProjectionTexture
+    from kernelkit.processing import filter_into
+
+    pgs, vg = workloads.config_geometry("small")
+    x = torch.from_numpy(workloads.ellipsoid_phantom(vg.shape, n=10,
+                                                     seed=0)).cuda()
+    A = kk.XrayTransform(pgs, vg)
+    y = A(x)
+    v = torch.empty_like(x)
+    stream = torch.cuda.current_stream()
+    tex = ProjectionTexture(*y.shape, like=y)
+    tex.zero()
+    bptor = kk.BackProjector()
+    bptor.projection_geometry = pgs
+    bptor.volume_geometry = vg
+    bptor.volume = v
+
+    def fdk():
+        filter_into(y, tex, geoms=pgs, filter="ramlak")
+        bptor._textures = tex
+        bptor()
+
+    def sirt():
+        kk.sirt(y, pgs, vg, iters=sirt_iters, verbose=False, return_gpu=True)
+
+    out = {}
+    for name, fn, reps in (("fp", lambda: A(x, out=y), 5),
+                           ("bp", lambda: A.T(y, out=v), 5),
+                           ("fdk", fdk, 5), (f"sirt_{sirt_iters}", sirt, 2)):
+        fn()
+        out[f"{name}_ms"] = round(_event_ms(fn, stream, reps=reps), 3)
+    return out
+
+
+def cpu_small(workers=16, sirt_iters=20, single_sirt_iters=20, gpu=True):
     """CPU baseline at the small workload (128^3, 180 angles, 160^2
     detector) in full: FP, BP, Ram-Lak FDK and ``sirt_iters`` SIRT
     iterations of the FP64 oracle on ``workers`` host processes, and the
-    same on one process (SIRT there for ``single_sirt_iters`` iterations,
-    with their setup: the two normalisation passes)."""
+    same on one process (SIRT there for ``single_sirt_iters`` iterations);
+    SIRT times include their setup (the two normalisation passes).  With
+    ``gpu``, the same jobs on the GPU beside them."""
     import numpy as np
 
     from kernelkit import workloads
@@ -360,6 +404,8 @@ def cpu_small(workers=16, sirt_iters=20, single_sirt_iters=1):
             "fdk_s": round(t["fdk"], 2),
             f"sirt_{iters}_iterations_s": round(t["sirt"], 2)}
         _log(f"cpu small, {cores} cores: {out[f'{cores}_cores']}")
+    if gpu:
+        out["gpu_ms"] = gpu_small(sirt_iters)
     return out
 
 
