@@ -178,6 +178,19 @@ __device__ __forceinline__ void flush_run(float (*g)[kJC][64], int32_t lane,
   }
 }
 
+// Ends a run whose tap index is the float t: flush_run when t is a tap
+// row of the run [-1, kJB), else the sums are dropped (runs outside the
+// b-run, the "no run yet" marker -2, non-finite coordinates).
+__device__ __forceinline__ void close_run(float (*g)[kJC][64], int32_t lane,
+                                          float t, float* S, float* T) {
+  if (t >= -1.f && t < static_cast<float>(kJB)) {
+    flush_run(g, lane, static_cast<int32_t>(t), S, T);
+  } else {
+#pragma unroll
+    for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
+  }
+}
+
 // Value of lane `l` of a wavefront (l wave-uniform), in scalar registers.
 __device__ __forceinline__ int32_t bcast(int32_t v, int32_t l) {
   return __builtin_amdgcn_readlane(v, l);
@@ -464,7 +477,13 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         // to the lane's LDS words when the index changes (along a column
         // it rarely does): S = sum of y w_c, T = sum of y w_c fb, so the
         // taps t and t + 1 receive S - T and T
-        int32_t tcur = -2;  // no run yet
+        // The run's tap index is kept as a float (the floor of x_b): a
+        // new index closes the run, whose sums reach the LDS words only if
+        // the index lies in [-1, kJB) and are dropped otherwise.  Rows
+        // outside the lane's tents add zeros; a non-finite coordinate (a
+        // ray parallel to the slices) starts a run of its own that is
+        // dropped.
+        float tcur = -2.f;  // no run yet
         float S[kJC], T[kJC];
 #pragma unroll
         for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
@@ -472,15 +491,11 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         for (int32_t r = rl; r <= rh; ++r, drf += 1.f) {
           const float sc = arf * __builtin_amdgcn_rcpf(fmaf(drf, vaf, den0));
           const float xc = fmaf(fmaf(drf, kvc, nc0), sc, oc);
-          // the lane's tents are zero outside x_c in (-1, kJC)
-          if (!(fabsf(xc - 0.5f) < 1.5f)) continue;
           const float xb = fmaf(fmaf(drf, kvb, nb0), sc, ob);
           const float fl = floorf(xb);
-          if (!(fl >= -1.f && fl < static_cast<float>(kJB))) continue;
-          const int32_t t0 = static_cast<int32_t>(fl);
-          if (t0 != tcur) {
-            if (tcur >= -1) flush_run(g, lane, tcur, S, T);
-            tcur = t0;
+          if (fl != tcur) {
+            close_run(g, lane, tcur, S, T);
+            tcur = fl;
           }
           const float yv = ycol[r];
           const float yfb = yv * (xb - fl);
@@ -491,7 +506,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
           T[0] = fmaf(yfb, w0, T[0]);
           T[1] = fmaf(yfb, w1, T[1]);
         }
-        if (tcur >= -1) flush_run(g, lane, tcur, S, T);
+        close_run(g, lane, tcur, S, T);
       }
     }
   }

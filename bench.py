@@ -626,7 +626,9 @@ def _batch_vs_plain(problems, ys, iters, stream):
                           for pg, vg in problems)
     return {"batched_gups": round(upd / (t_graph * 1e-3) / 1e9, 1),
             "per_problem_gups": round(upd / (t_plain * 1e-3) / 1e9, 1),
-            "speedup": round(t_plain / t_graph, 3)}
+            "speedup": round(t_plain / t_graph, 3),
+            # the tiles the tuner picks for this class alone
+            "tiles": {"fp": st_cfg_repr(B, "fp"), "bp": st_cfg_repr(B, "bp")}}
 
 
 def st_cfg_repr(B, which):
