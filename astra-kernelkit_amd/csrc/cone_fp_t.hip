@@ -427,8 +427,11 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         int_range(cmin, cmax, A.cols, c_lo, c_hi);
         int_range(rmin, rmax, A.rows, r_lo, r_hi);
       }
-      // columns in a wave-uniform loop (the union of the lanes' ranges),
-      // rows per lane.  Sample coordinates: float64 at an anchor pixel
+      // columns in a loop of wave-uniform length, each lane walking its
+      // own column range from its first column (a tilted rotation axis
+      // shifts the lanes' ranges against each other: a loop over their
+      // union would leave a lane idle for the difference), rows per lane.
+      // Sample coordinates: float64 at an anchor pixel
       // (the wave's first column, the lane's first row), float32 offsets
       // from it.  With D = q + col U + row V (kk_fp_ray's terms) and
       // t = D_b / D_a at the anchor, x_b = arel D_b / D_a + tau_b moves by
@@ -437,7 +440,6 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       // voxel.  Weights are the tents 1 - |x - centre| of the forward
       // kernel's bilinear taps.
       const int32_t wc_lo = kk_wave_min(c_lo <= c_hi ? c_lo : A.cols);
-      const int32_t wc_hi = kk_wave_max(c_lo <= c_hi ? c_hi : -1);
       const double fwc = static_cast<double>(wc_lo);
       const double fr0 = static_cast<double>(r_lo);
       const double Da0 = fma(fr0, P.va, fma(fwc, P.ua, P.qa));
@@ -463,7 +465,11 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       // the numerator is affine in u with slope vb - tb va, and the
       // denominator with slope va (c alike).
       const float kvb = fmaf(-tbf, vaf, vbf), kvc = fmaf(-tcf, vaf, vcf);
-      for (int32_t col = wc_lo; col <= wc_hi; ++col) {
+      const int32_t ncol = (anchor_ok && c_lo <= c_hi) ? c_hi - c_lo + 1 : 0;
+      const int32_t wncol = kk_wave_max(ncol);
+      for (int32_t j = 0; j < wncol; ++j) {
+        const int32_t col = c_lo + j;  // this lane's column (in range if
+                                       // j < ncol)
         const float* ycol = img + col * cstride;
         const float dcf = static_cast<float>(col - wc_lo);
         const float dAc = dcf * uaf;
@@ -471,8 +477,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         const float nc0 = fmaf(-tcf, dAc, dcf * ucf);
         const float den0 = da0 + dAc;
         const int32_t rl = r_lo;
-        const int32_t rh = (anchor_ok && col >= c_lo && col <= c_hi) ? r_hi
-                                                                    : r_lo - 1;
+        const int32_t rh = j < ncol ? r_hi : r_lo - 1;
         // b-tap sums over a run of samples with the same tap index, added
         // to the lane's LDS words when the index changes (along a column
         // it rarely does): S = sum of y w_c, T = sum of y w_c fb, so the
