@@ -467,6 +467,20 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       const float kvb = fmaf(-tbf, vaf, vbf), kvc = fmaf(-tcf, vaf, vcf);
       const int32_t ncol = (anchor_ok && c_lo <= c_hi) ? c_hi - c_lo + 1 : 0;
       const int32_t wncol = kk_wave_max(ncol);
+      // b-tap sums over a run of samples with the same tap index, added to
+      // the lane's LDS words when the index changes: S = sum of y w_c,
+      // T = sum of y w_c fb, so the taps t and t + 1 receive S - T and T.
+      // A run stays open from column to column while the index holds (the
+      // sums of one index add up whatever the column), so along the walk
+      // a run closes about once per b voxel crossed.  The index is kept as
+      // a float (the floor of x_b); a run reaches the LDS words only if
+      // its index lies in [-1, kJB) and is dropped otherwise.  Rows
+      // outside the lane's tents add zeros; a non-finite coordinate (a ray
+      // parallel to the slices) starts a run of its own that is dropped.
+      float tcur = -2.f;  // no run yet
+      float S[kJC], T[kJC];
+#pragma unroll
+      for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
       for (int32_t j = 0; j < wncol; ++j) {
         const int32_t col = c_lo + j;  // this lane's column (in range if
                                        // j < ncol)
@@ -478,20 +492,6 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
         const float den0 = da0 + dAc;
         const int32_t rl = r_lo;
         const int32_t rh = j < ncol ? r_hi : r_lo - 1;
-        // b-tap sums over a run of samples with the same tap index, added
-        // to the lane's LDS words when the index changes (along a column
-        // it rarely does): S = sum of y w_c, T = sum of y w_c fb, so the
-        // taps t and t + 1 receive S - T and T
-        // The run's tap index is kept as a float (the floor of x_b): a
-        // new index closes the run, whose sums reach the LDS words only if
-        // the index lies in [-1, kJB) and are dropped otherwise.  Rows
-        // outside the lane's tents add zeros; a non-finite coordinate (a
-        // ray parallel to the slices) starts a run of its own that is
-        // dropped.
-        float tcur = -2.f;  // no run yet
-        float S[kJC], T[kJC];
-#pragma unroll
-        for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
         float drf = 0.f;
         for (int32_t r = rl; r <= rh; ++r, drf += 1.f) {
           const float sc = arf * __builtin_amdgcn_rcpf(fmaf(drf, vaf, den0));
@@ -511,8 +511,8 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
           T[0] = fmaf(yfb, w0, T[0]);
           T[1] = fmaf(yfb, w1, T[1]);
         }
-        close_run(g, lane, tcur, S, T);
       }
+      close_run(g, lane, tcur, S, T);
     }
   }
 
