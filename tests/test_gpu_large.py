@@ -1,8 +1,9 @@
 """The largest workload (1024^3 volume, 1536^2 detector, seeded wobble and
 drift) on the GPU, against the oracle on bounded samples: the forward
-projection on detector row bands of two angles, the backprojection on a
-z-slab of the volume (both exact restrictions: each ray and each voxel is
-computed independently)."""
+projection on detector row bands of eight angles (including the three
+around the 45-degree ray-axis switch), the backprojection of those angles
+on z-slabs of the volume (both exact restrictions: each ray and each voxel
+is computed independently)."""
 import numpy as np
 import pytest
 
@@ -30,7 +31,9 @@ def _band(g, r0, band):
 @pytest.fixture(scope="module")
 def large():
     pgs, vg = workloads.config_geometry("large")
-    return [pgs[0], pgs[611]], vg
+    # 0.25 degrees apart: 179-181 straddle 45 degrees (the x/y ray-axis
+    # switch under the wobble), the others spread over the orbit
+    return [pgs[k] for k in (0, 179, 180, 181, 611, 719, 1080, 1439)], vg
 
 
 def test_large_fp_bands(large, device):
@@ -59,7 +62,7 @@ def test_large_bp_slab(large, device):
     import torch
     pgs, vg = large
     rng = np.random.default_rng(0)
-    proj = rng.uniform(size=(2, 1536, 1536)).astype(np.float32)
+    proj = rng.uniform(size=(len(pgs), 1536, 1536)).astype(np.float32)
     A = kk.XrayTransform(pgs, vg)
     v = A.T(torch.from_numpy(proj).cuda())
     n = vg.shape[2]
