@@ -13,9 +13,11 @@ arrays or :class:`kernelkit.DeviceArray` (memory from the library's own
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes
 import os
 import sys
+import threading
 from pathlib import Path
 
 import numpy as np
@@ -114,6 +116,21 @@ _SIGNATURES = {
     "kk_batch_add_volume_texture": (ctypes.c_int, [
         _vp, _vp, _c_i32p, _c_i64p, ctypes.c_int32, _vp]),
     "kk_batch_residual": (ctypes.c_int, [_vp, _vp]),
+    "kk_device_info": (ctypes.c_int, [_c_i32p, ctypes.c_char_p,
+                                      ctypes.c_int32]),
+    "kk_stream_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "kk_stream_destroy": (ctypes.c_int, [_vp]),
+    "kk_stream_wait_event": (ctypes.c_int, [_vp, _vp]),
+    "kk_event_create": (ctypes.c_int, [ctypes.POINTER(_vp)]),
+    "kk_event_destroy": (ctypes.c_int, [_vp]),
+    "kk_event_record": (ctypes.c_int, [_vp, _vp]),
+    "kk_event_synchronize": (ctypes.c_int, [_vp]),
+    "kk_event_elapsed_ms": (ctypes.c_int, [_vp, _vp,
+                                           ctypes.POINTER(ctypes.c_float)]),
+    "kk_graph_begin_capture": (ctypes.c_int, [_vp]),
+    "kk_graph_end_capture": (ctypes.c_int, [_vp, ctypes.POINTER(_vp)]),
+    "kk_graph_launch": (ctypes.c_int, [_vp, _vp]),
+    "kk_graph_destroy": (ctypes.c_int, [_vp]),
     "kk_batch_volume_textures": (ctypes.c_int, [_vp, _vp]),
     "kk_probe_l2_read": (ctypes.c_int, [_vp, ctypes.c_int64, ctypes.c_int32,
                                         ctypes.c_int32, _vp, _vp]),
@@ -258,12 +275,33 @@ def i32_ptr(arr: np.ndarray):
     return arr.ctypes.data_as(_c_i32p)
 
 
+_local = threading.local()
+
+
+@contextlib.contextmanager
+def use_stream(stream):
+    """Make ``stream`` the default of :func:`stream_ptr` in this thread
+    (graph capture records the library's launches on its own stream)."""
+    prev = getattr(_local, "stream", None)
+    _local.stream = stream_ptr(stream)
+    try:
+        yield
+    finally:
+        _local.stream = prev
+
+
 def stream_ptr(stream=None) -> ctypes.c_void_p:
     """Raw hipStream_t of ``stream``: a torch stream, an object carrying
     the handle as ``cuda_stream`` or ``ptr`` (CuPy's), or an integer.  By
-    default torch's current stream when PyTorch is loaded and has set up
-    its HIP runtime, else the null stream (torch is never imported here)."""
+    default the stream set by :func:`use_stream`, else torch's current
+    stream when PyTorch is loaded and has set up its HIP runtime, else the
+    null stream (torch is never imported here)."""
+    if isinstance(stream, ctypes.c_void_p):
+        return stream
     if stream is None:
+        override = getattr(_local, "stream", None)
+        if override is not None:
+            return override
         torch = sys.modules.get("torch")
         if torch is not None and torch.cuda.is_initialized():
             stream = torch.cuda.current_stream()
@@ -342,6 +380,122 @@ class Filter:
         if ptr is not None and ptr.value and _lib is not None:
             try:
                 _lib.kk_filter_destroy(ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self._ptr = None
+
+
+def device_info() -> tuple:
+    """``(compute units, device name)`` of the current device (the
+    architecture name where the runtime reports no marketing name)."""
+    cus = ctypes.c_int32(0)
+    name = ctypes.create_string_buffer(256)
+    check(lib().kk_device_info(ctypes.byref(cus), name, len(name)))
+    return int(cus.value), name.value.decode(errors="replace")
+
+
+class Stream:
+    """Owner of a library stream (non-blocking, default priority)."""
+
+    def __init__(self):
+        ptr = _vp()
+        check(lib().kk_stream_create(ctypes.byref(ptr)))
+        self._ptr = ptr
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._ptr
+
+    def wait(self, event: "Event") -> None:
+        check(lib().kk_stream_wait_event(self._ptr, event.ptr))
+
+    def synchronize(self) -> None:
+        check(lib().kk_stream_synchronize(self._ptr))
+
+    def __del__(self):
+        ptr = getattr(self, "_ptr", None)
+        if ptr is not None and ptr.value and _lib is not None:
+            try:
+                _lib.kk_stream_destroy(ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self._ptr = None
+
+
+class Event:
+    """Owner of a timing event."""
+
+    def __init__(self):
+        ptr = _vp()
+        check(lib().kk_event_create(ctypes.byref(ptr)))
+        self._ptr = ptr
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._ptr
+
+    def record(self, stream=None) -> None:
+        check(lib().kk_event_record(self._ptr, stream_ptr(stream)))
+
+    def synchronize(self) -> None:
+        check(lib().kk_event_synchronize(self._ptr))
+
+    def elapsed_ms(self, end: "Event") -> float:
+        ms = ctypes.c_float(0.0)
+        check(lib().kk_event_elapsed_ms(self._ptr, end.ptr,
+                                        ctypes.byref(ms)))
+        return float(ms.value)
+
+    def __del__(self):
+        ptr = getattr(self, "_ptr", None)
+        if ptr is not None and ptr.value and _lib is not None:
+            try:
+                _lib.kk_event_destroy(ptr)
+            except Exception:  # interpreter shutdown
+                pass
+            self._ptr = None
+
+
+class Graph:
+    """An executable HIP graph of library launches.
+
+    ``Graph.capture(fn)`` runs ``fn`` with :func:`stream_ptr` defaulting to
+    a fresh capturing stream, so every launch it makes through this binding
+    is recorded instead of run; ``fn`` must not allocate device memory,
+    synchronise or launch on an explicit stream.  ``replay`` launches the
+    recorded work on ``stream`` (default: :func:`stream_ptr`)."""
+
+    def __init__(self, exec_ptr):
+        self._ptr = exec_ptr
+
+    @classmethod
+    def capture(cls, fn) -> "Graph":
+        s = Stream()
+        check(lib().kk_graph_begin_capture(s.ptr))
+        try:
+            with use_stream(s.ptr):
+                fn()
+        except BaseException:
+            dead = _vp()
+            if lib().kk_graph_end_capture(s.ptr, ctypes.byref(dead)) == KK_OK:
+                lib().kk_graph_destroy(dead)
+            raise
+        exec_ptr = _vp()
+        check(lib().kk_graph_end_capture(s.ptr, ctypes.byref(exec_ptr)))
+        return cls(exec_ptr)
+
+    @property
+    def ptr(self) -> ctypes.c_void_p:
+        return self._ptr
+
+    def replay(self, stream=None) -> None:
+        check(lib().kk_graph_launch(self._ptr, stream_ptr(stream)))
+
+    def __del__(self):
+        ptr = getattr(self, "_ptr", None)
+        if ptr is not None and ptr.value and _lib is not None:
+            try:
+                _lib.kk_graph_destroy(ptr)
             except Exception:  # interpreter shutdown
                 pass
             self._ptr = None

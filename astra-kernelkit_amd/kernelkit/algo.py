@@ -245,19 +245,13 @@ class _SirtState:
         self.xtex.update(self.x)
 
     def capture(self):
-        """Record one iteration in a HIP graph (``torch.cuda.graph``): its
-        four launches replay as one.  Nothing is allocated or cleared inside
-        the iteration (every launch writes all it reads next), so the graph
-        holds kernel nodes only."""
-        import torch
-        g = torch.cuda.CUDAGraph()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.graph(g, stream=s):
-            self.iterate()
-        torch.cuda.current_stream().wait_stream(s)
-        self.graph = g
-        return g
+        """Record one iteration in a HIP graph (the library's own stream
+        capture, :class:`kernelkit._native.Graph`): its four launches replay
+        as one.  Nothing is allocated or cleared inside the iteration (every
+        launch writes all it reads next), so the graph holds kernel nodes
+        only."""
+        self.graph = _native.Graph.capture(self.iterate)
+        return self.graph
 
     def step(self):
         """One iteration: a graph replay once captured, else eager."""

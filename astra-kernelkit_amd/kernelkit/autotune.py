@@ -26,9 +26,7 @@ GRIDS_PER_CU = (0, 4, 8)   # persistent workgroups per CU (0: one per tile)
 
 def _grids():
     """Candidate persistent grid sizes for the current device."""
-    import torch
-    n_cu = torch.cuda.get_device_properties(
-        torch.cuda.current_device()).multi_processor_count
+    n_cu, _ = _native.device_info()
     return tuple(g * n_cu for g in GRIDS_PER_CU)
 
 
@@ -67,23 +65,19 @@ class BatchTuner:
     def _key(signature) -> str:
         """Cache key: the batch signature, the library build and the device
         (timings do not carry over between builds or devices)."""
-        import torch
-        dev = torch.cuda.get_device_name(torch.cuda.current_device())
-        return repr((_native.build_id(), dev, signature))
+        return repr((_native.build_id(), _native.device_info(), signature))
 
     def _time(self, launch) -> float:
-        import torch
         launch()                       # warm-up
-        stream = torch.cuda.current_stream()
+        stream = _native.stream_ptr()
+        a, b = _native.Event(), _native.Event()
         ts = []
         for _ in range(self.reps):
-            a = torch.cuda.Event(enable_timing=True)
-            b = torch.cuda.Event(enable_timing=True)
             a.record(stream)
             launch()
             b.record(stream)
             b.synchronize()
-            ts.append(a.elapsed_time(b))
+            ts.append(a.elapsed_ms(b))
         return min(ts)
 
     def _tune(self, handle, which, other_tile):

@@ -16,7 +16,7 @@ from __future__ import annotations
 from typing import Sequence
 
 
-from kernelkit import _native
+from kernelkit import _native, device
 from kernelkit.autotune import BatchTuner
 from kernelkit.kernel import ProjectionTexture, as_device_array
 from kernelkit.kernels.cone_fp import _logical_strides, detector_shape
@@ -27,8 +27,10 @@ class ConeBatch:
     """Independent cone-beam problems ``[(projection_geometry,
     volume_geometry), ...]`` run together.
 
-    Volumes are ``(x, y, z)`` float32 device tensors and projections
-    ``(angle, row, col)`` ones (the default axes of ``XrayTransform``).
+    Volumes are ``(x, y, z)`` float32 device arrays (torch tensors,
+    :class:`kernelkit.DeviceArray` or ``__cuda_array_interface__`` arrays)
+    and projections ``(angle, row, col)`` ones (the default axes of
+    ``XrayTransform``); new outputs are of the inputs' kind.
     """
 
     def __init__(self, problems: Sequence[tuple], tuner: BatchTuner = None):
@@ -50,13 +52,12 @@ class ConeBatch:
            tune: bool = True) -> list:
         """``projections[p] = A_p volumes[p]`` for every problem, one launch
         (after the per-problem texture conversions)."""
-        import torch
         vols = [as_device_array(v) for v in volumes]
         self._check(vols, self.domain_shapes, "volume")
         if projections is None:
-            projections = [torch.empty(s, dtype=torch.float32,
-                                       device=vols[0].device)
+            projections = [device.empty(s, like=vols[0])
                            for s in self.range_shapes]
+        projections = [as_device_array(p) for p in projections]
         self._check(projections, self.range_shapes, "projection")
         for op, v in zip(self.ops, vols):
             op.projector.volume = v          # builds the frame textures
@@ -88,13 +89,12 @@ class ConeBatch:
            tune: bool = True) -> list:
         """``volumes[p] = A_p^* projections[p]`` (the voxel-driven
         backprojector) for every problem, one launch."""
-        import torch
         projs = [as_device_array(p) for p in projections]
         self._check(projs, self.range_shapes, "projection")
         if volumes is None:
-            volumes = [torch.empty(s, dtype=torch.float32,
-                                   device=projs[0].device)
+            volumes = [device.empty(s, like=projs[0])
                        for s in self.domain_shapes]
+        volumes = [as_device_array(v) for v in volumes]
         self._check(volumes, self.domain_shapes, "volume")
         for op, y in zip(self.ops, projs):
             op.backprojector.projections = y  # builds the BP textures
@@ -135,7 +135,7 @@ class ConeBatch:
              max_constraint: float = None, graph: bool = True,
              tune: bool = True) -> list:
         """``iters`` SIRT iterations of every problem (as ``algo.sirt``),
-        two launches per iteration for the whole batch; with ``graph`` the
+        four launches per iteration for the whole batch; with ``graph`` the
         iteration loop is captured once in a HIP graph and replayed.
         Returns the device volumes."""
         st = self.sirt_state(projections, mask, min_constraint,
@@ -167,13 +167,11 @@ class BatchSirt:
 
     def __init__(self, batch: ConeBatch, projections, mask, min_constraint,
                  max_constraint):
-        import torch
         self.batch = batch
         lib = _native.lib()
         st = _native.stream_ptr()
         ys = [as_device_array(p) for p in projections]
         batch._check(ys, batch.range_shapes, "projection")
-        dev = ys[0].device
         if mask is not None:
             masks = [as_device_array(m) for m in mask]
             batch._check(masks, batch.domain_shapes, "mask")
@@ -188,26 +186,26 @@ class BatchSirt:
             shape = tuple(int(s) for s in vg.shape)
             n, rows, cols = y.shape
             # C = 1/BP(1), R = 1/FP(1), inf -> 0 (plain single launches)
-            C = op.T(torch.ones_like(y))
-            R = op(torch.ones(shape, dtype=torch.float32, device=dev))
+            C = op.T(device.full(y.shape, 1.0, like=y))
+            R = op(device.full(shape, 1.0, like=y))
             _native.check(lib.kk_recip_or_zero(_native._vp(C.data_ptr()),
                                                C.numel(), st))
             _native.check(lib.kk_recip_or_zero(_native._vp(R.data_ptr()),
                                                R.numel(), st))
-            x = torch.zeros(shape, dtype=torch.float32, device=dev)
+            x = device.zeros(shape, like=y)
             # the state's own FP texture of x: the operator's projector
             # texture is rebuilt by any later call of the operator
             fk = op.projector.kernel
             op.projector._require_geometry()
             xtex = fk.volume_texture(x, vg)
-            rtex = ProjectionTexture(n, rows, cols, device=dev)
+            rtex = ProjectionTexture(n, rows, cols, like=y)
             rtex.zero()
             bk = op.backprojector.kernel
             # residual in two steps: the batch FP into a buffer, then one
             # coalesced (p - y) R pass per problem into its BP texture (the
             # fused epilogue, kk_batch_add_sirt_fp, reads y and R one
             # detector row per lane, which costs more on MI355X)
-            p = torch.empty((n, rows, cols), dtype=torch.float32, device=dev)
+            p = device.empty((n, rows, cols), like=y)
             _native.check(lib.kk_batch_add_fp(
                 b.ptr, fk._plan.ptr, _native._vp(xtex.ptr),
                 _native._vp(p.data_ptr()), _native.i64(p.stride()), rows,
@@ -276,18 +274,16 @@ class BatchSirt:
         _native.check(lib.kk_batch_volume_textures(h, st))
 
     def capture(self, iters: int):
-        """Capture ``iters`` iterations in a HIP graph (torch.cuda.graph);
-        replaying it continues from the current ``x``."""
-        import torch
+        """Capture ``iters`` iterations in a HIP graph (the library's own
+        stream capture, :class:`kernelkit._native.Graph`); replaying it
+        continues from the current ``x``."""
         if self.cfg_fp is None:
             self.configure(tune=False)
-        g = torch.cuda.CUDAGraph()
-        s = torch.cuda.Stream()
-        s.wait_stream(torch.cuda.current_stream())
-        with torch.cuda.graph(g, stream=s):
+
+        def body():
             for _ in range(iters):
                 self.iterate()
-        torch.cuda.current_stream().wait_stream(s)
+        g = _native.Graph.capture(body)
         self._graph = (iters, g)
         return g
 

@@ -130,6 +130,31 @@ int kk_copy_strided(const void* src, const int64_t src_strides[3], void* dst,
                     const int64_t dst_strides[3], const int64_t shape[3],
                     int32_t elem_bytes, void* stream);
 
+/* ---- streams, events, graphs (callers without PyTorch) -------------- */
+
+/* The reference records projector calls in a CUDA graph through CuPy's
+ * stream capture (example/cuda/graph.py:46-51).  These calls give the
+ * drop-in and C callers streams (default priority, non-blocking), timing
+ * events and stream capture without PyTorch.  kk_graph_end_capture
+ * instantiates the captured work and returns an executable graph, replayed
+ * with kk_graph_launch on any stream; the null stream cannot capture.
+ * kk_device_info: the current device's compute-unit count and name, or
+ * its architecture name where the runtime reports no marketing name
+ * (name_len bytes including the terminator; either output may be null). */
+int kk_device_info(int32_t* cu_count, char* name, int32_t name_len);
+int kk_stream_create(void** stream);
+int kk_stream_destroy(void* stream);
+int kk_stream_wait_event(void* stream, void* event);
+int kk_event_create(void** event);
+int kk_event_destroy(void* event);
+int kk_event_record(void* event, void* stream);
+int kk_event_synchronize(void* event);
+int kk_event_elapsed_ms(void* start, void* stop, float* ms);
+int kk_graph_begin_capture(void* stream);
+int kk_graph_end_capture(void* stream, void** graph_exec);
+int kk_graph_launch(void* graph_exec, void* stream);
+int kk_graph_destroy(void* graph_exec);
+
 /* Ray-driven Joseph forward projection, all angles in one launch
  * (cone_fp<DIR_X|Y|Z>, kernelkit/cuda/cone_fp.cu:73-207, launched by
  * kernels/cone_fp.py:195-262).  vol_tex: FP texture holding at least the

@@ -1,10 +1,12 @@
 """The operators without PyTorch.
 
 The reference computes on CuPy arrays (``kernelkit/kernel.py:47-119``); the
-drop-in takes any device array.  One test runs FP, BP, FDK, SIRT and the
-exact transpose in a child process in which ``import torch`` fails, with
-host (numpy) inputs and :class:`kernelkit.DeviceArray` inputs, and checks
-them against the oracle; another feeds arrays that only expose
+drop-in takes any device array.  One test runs FP, BP, FDK, SIRT (eager and
+replayed from a HIP graph), the exact transpose and a batch of two problems
+(FP, BP, tuned tile shapes, graph-replayed SIRT) in a child process in which
+``import torch`` fails, with host (numpy) inputs and
+:class:`kernelkit.DeviceArray` inputs, and checks them against the oracle;
+another feeds arrays that only expose
 ``__cuda_array_interface__`` (as CuPy arrays do) and checks the results
 against the torch path.
 """
@@ -80,6 +82,43 @@ ys = kk.XrayTransform(pgs, vg_sub)(sub).get()
 ys_ref = oracle.cone_fp(vol[:, 1:29].astype(np.float64), pgs, vg_sub)
 res["view_fp"] = (rel(ys, ys_ref), point(ys, ys_ref))
 
+# a batch of two problems of different shapes: one launch per operator,
+# tile shapes timed with the library's events, SIRT replayed from the
+# library's own graph capture
+from kernelkit.autotune import BatchTuner
+from kernelkit.batch import ConeBatch
+cus, name = kk._native.device_info()
+assert cus > 0 and name, (cus, name)
+pgs1 = pgs[::2]
+vol1 = np.ascontiguousarray(vol[:, 1:29])
+B = ConeBatch([(pgs, vg), (pgs1, vg_sub)], tuner=BatchTuner(reps=1))
+yb = B.fp([dv, kk.device.from_host(vol1)], tune=True)
+assert all(isinstance(t, kk.DeviceArray) for t in yb)
+y1_ref = oracle.cone_fp(vol1.astype(np.float64), pgs1, vg_sub)
+for k, ref in enumerate((y_ref, y1_ref)):
+    res[f"batch_fp{k}"] = (rel(yb[k].get(), ref), point(yb[k].get(), ref))
+xb2 = B.bp(yb, tune=True)
+for k, (g, vgk) in enumerate(((pgs, vg), (pgs1, vg_sub))):
+    ref = oracle.cone_bp(yb[k].get().astype(np.float64), g, vgk)
+    res[f"batch_bp{k}"] = (rel(xb2[k].get(), ref), point(xb2[k].get(), ref))
+y1 = yb[1].get()
+st = B.sirt_state([kk.device.from_host(y), kk.device.from_host(y1)])
+st.configure(tune=True)
+st.run(2, graph=True)
+st.run(2, graph=True)          # a second replay continues iterating
+s1_ref = oracle.sirt(y1.astype(np.float64), pgs1, vg_sub, iters=4)
+for k, ref in enumerate((s_ref, s1_ref)):
+    res[f"batch_sirt{k}"] = (rel(st.x[k].get(), ref),
+                             point(st.x[k].get(), ref))
+sg = kk.sirt(y, pgs, vg, iters=4, verbose=False, graph=True)
+res["sirt_graph"] = (rel(sg, s_ref), point(sg, s_ref))
+a, b = kk._native.Event(), kk._native.Event()
+a.record()
+A(dv, out=dy)
+b.record()
+b.synchronize()
+assert a.elapsed_ms(b) > 0.0
+
 assert sys.modules.get("torch") is None, "torch was imported"
 print("RESULT " + json.dumps(res), flush=True)
 """
@@ -99,7 +138,9 @@ def test_without_torch_against_oracle(tmp_path):
     res = json.loads(line[-1][len("RESULT "):])
     print(res)
     assert set(res) == {"fp", "bp", "fdk", "sirt", "op_fp", "transpose",
-                        "op_bp", "view_fp"}
+                        "op_bp", "view_fp", "batch_fp0", "batch_fp1",
+                        "batch_bp0", "batch_bp1", "batch_sirt0",
+                        "batch_sirt1", "sirt_graph"}
     for name, (e, p) in res.items():
         assert e <= 1e-5, (name, e)
         assert p <= 1e-4, (name, p)

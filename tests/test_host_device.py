@@ -82,3 +82,31 @@ def test_broadcast_helpers_and_kinds():
     assert device.dtype_name(bm) == "float32"
     assert device._is_device(m) and not device._is_device(np.zeros(3))
     assert kk.kernel.is_device_array(_FakeCai((2,)))
+
+
+def test_use_stream_sets_the_default_stream():
+    """Graph capture records the library's launches by making its capturing
+    stream the default of ``stream_ptr`` for the duration of the capture;
+    explicit streams still win and the default comes back afterwards."""
+    import ctypes
+    import threading
+
+    from kernelkit import _native
+    before = _native.stream_ptr().value
+    with _native.use_stream(0x1000):
+        assert _native.stream_ptr().value == 0x1000
+        assert _native.stream_ptr(0x2000).value == 0x2000
+        with _native.use_stream(ctypes.c_void_p(0x3000)):
+            assert _native.stream_ptr().value == 0x3000
+        assert _native.stream_ptr().value == 0x1000
+        seen = []
+        t = threading.Thread(target=lambda: seen.append(
+            _native.stream_ptr().value))
+        t.start()
+        t.join()
+        assert seen == [before]          # other threads are not affected
+    assert _native.stream_ptr().value == before
+    with pytest.raises(RuntimeError):
+        with _native.use_stream(0x1000):
+            raise RuntimeError("inside")
+    assert _native.stream_ptr().value == before
