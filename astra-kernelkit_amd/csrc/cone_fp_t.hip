@@ -115,28 +115,14 @@ __global__ __launch_bounds__(256) void ray_weight_kernel(
   }
 }
 
-// Detector position (col, row) of the ray from the source through the
-// source-relative point X (frame components): q + col U + row V = lam X;
-// ok iff finite with lam > 0.
-__device__ __forceinline__ bool project(const kk::FpAngle& P, double xa,
-                                        double xb, double xc, double& col,
-                                        double& row) {
-  const double vx0 = P.vb * xc - P.vc * xb;
-  const double vx1 = P.vc * xa - P.va * xc;
-  const double vx2 = P.va * xb - P.vb * xa;
-  const double qx0 = P.qb * xc - P.qc * xb;
-  const double qx1 = P.qc * xa - P.qa * xc;
-  const double qx2 = P.qa * xb - P.qb * xa;
-  const double vq0 = P.vb * P.qc - P.vc * P.qb;
-  const double vq1 = P.vc * P.qa - P.va * P.qc;
-  const double vq2 = P.va * P.qb - P.vb * P.qa;
-  // one reciprocal for the three quotients (a box corner: its float64
-  // rounding is far below the 1e-6 voxel widening of the box)
-  const double inv = 1.0 / (P.ua * vx0 + P.ub * vx1 + P.uc * vx2);
-  col = -(P.qa * vx0 + P.qb * vx1 + P.qc * vx2) * inv;
-  row = -(P.ua * qx0 + P.ub * qx1 + P.uc * qx2) * inv;
-  const double lam = (P.ua * vq0 + P.ub * vq1 + P.uc * vq2) * inv;
-  return isfinite(col) && isfinite(row) && lam > 0.0;
+// 1 / d in float64: the hardware estimate and two Newton steps (a few
+// units in the last place; non-finite for d = 0).
+__device__ __forceinline__ double rcp64(double d) {
+  double r = __builtin_amdgcn_rcp(d);
+  double e = fma(-d, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-d, r, 1.0);
+  return fma(r, e, r);
 }
 
 // Integer interval [ceil(lo), floor(hi)] clipped to [0, n - 1].
@@ -165,47 +151,35 @@ __device__ __forceinline__ void tent_pair(float x, float y, float* colsum) {
   colsum[1] = fmaf(y, __builtin_amdgcn_fmed3f(0.f, x, 2.f - x), colsum[1]);
 }
 
-// Adds a run of samples with b tap index t (in [-1, kJB)) to the lane's
-// b-tap sums: S - T to row t and T to row t + 1, where they are inside the
-// b-run; clears the run.
-__device__ __forceinline__ void flush_run(float (*g)[kJC][64], int32_t lane,
-                                          int32_t t, float* S, float* T) {
-#pragma unroll
-  for (int cc = 0; cc < kJC; ++cc) {
-    if (t >= 0) g[t][cc][lane] += S[cc] - T[cc];
-    if (t + 1 < kJB) g[t + 1][cc][lane] += T[cc];
-    S[cc] = T[cc] = 0.f;
-  }
-}
-
-// Ends a run whose tap index is the float t: flush_run when t is a tap
-// row of the run [-1, kJB), else the sums are dropped (runs outside the
-// b-run, the "no run yet" marker -2, non-finite coordinates).
+// Ends a run of samples whose b tap index is the float t: S - T goes to
+// tap row t and T to row t + 1 of the lane's b-tap sums, then the run is
+// cleared.  g has a spare row on either side (rows -1 and kJB, never
+// written out), so the four read-modify-writes need no bounds branches
+// and are issued together (two paired LDS reads, two paired writes, one
+// round trip).  A run whose index is outside [-1, kJB) (outside the b-run,
+// the "no run yet" marker -2, non-finite coordinates) adds zeros to rows
+// -1 and 0.
 __device__ __forceinline__ void close_run(float (*g)[kJC][64], int32_t lane,
                                           float t, float* S, float* T) {
-  if (t >= -1.f && t < static_cast<float>(kJB)) {
-    flush_run(g, lane, static_cast<int32_t>(t), S, T);
-  } else {
+  static_assert(kJC == 2, "close_run covers two c voxels per lane");
+  const bool in = t >= -1.f && t < static_cast<float>(kJB);
+  const int32_t row = in ? static_cast<int32_t>(t) : -1;
+  float* p = &g[row][0][lane];
+  const float a0 = in ? S[0] - T[0] : 0.f, a1 = in ? S[1] - T[1] : 0.f;
+  const float b0 = in ? T[0] : 0.f, b1 = in ? T[1] : 0.f;
+  // rows are kJC * 64 words apart, c voxels 64 words
+  const float v0 = p[0], v1 = p[64], v2 = p[128], v3 = p[192];
+  p[0] = v0 + a0;
+  p[64] = v1 + a1;
+  p[128] = v2 + b0;
+  p[192] = v3 + b1;
 #pragma unroll
-    for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
-  }
+  for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
 }
 
 // Value of lane `l` of a wavefront (l wave-uniform), in scalar registers.
 __device__ __forceinline__ int32_t bcast(int32_t v, int32_t l) {
   return __builtin_amdgcn_readlane(v, l);
-}
-__device__ __forceinline__ float bcast(float v, int32_t l) {
-  return __builtin_bit_cast(
-      float, __builtin_amdgcn_readlane(__builtin_bit_cast(int32_t, v), l));
-}
-__device__ __forceinline__ double bcast(double v, int32_t l) {
-  const uint64_t u = __builtin_bit_cast(uint64_t, v);
-  const uint32_t lo = static_cast<uint32_t>(
-      __builtin_amdgcn_readlane(static_cast<int32_t>(u), l));
-  const uint32_t hi = static_cast<uint32_t>(
-      __builtin_amdgcn_readlane(static_cast<int32_t>(u >> 32), l));
-  return __builtin_bit_cast(double, (static_cast<uint64_t>(hi) << 32) | lo);
 }
 
 // Paths a launch compiles (its plan's angles need): the column-coherent
@@ -216,7 +190,7 @@ constexpr int kPathColumn = 1;
 constexpr int kPathGeneral = 2;
 
 template <int PATHS>
-__global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
+__global__ __launch_bounds__(kThreads, PATHS == 3 ? 5 : 6) void cone_fp_t_kernel(
     const kk::FpAngle* __restrict__ tab, FptArgs A) {
   int64_t bid = blockIdx.x;
   const int32_t tc = static_cast<int32_t>(bid % A.tiles_c);
@@ -237,11 +211,14 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
   const int64_t angle_elems = static_cast<int64_t>(cstride) * A.cols;
   // b-tap sums of this lane's voxels (row t: b voxel jw + t), indexed per
   // lane: every lane owns its words, so no atomics and no cross-lane
-  // ordering are needed; taps just outside the b-run are not added.  (A
-  // register array indexed by the wave-uniform tap row was compiled into
-  // private scratch memory.)
-  __shared__ float gacc[kWaves][kJB][kJC][64];
-  float(*g)[kJC][64] = gacc[threadIdx.x >> 6];
+  // ordering are needed; taps just outside the b-run are not written out.
+  // (A register array indexed by the wave-uniform tap row was compiled
+  // into private scratch memory.)  The general path's run flushes use a
+  // spare row on either side of the b-run (g[-1], g[kJB]); launches
+  // without that path keep the LDS smaller.
+  constexpr int kSpare = (PATHS & kPathGeneral) != 0 ? 1 : 0;
+  __shared__ float gacc[kWaves][kJB + 2 * kSpare][kJC][64];
+  float(*g)[kJC][64] = gacc[threadIdx.x >> 6] + kSpare;
   // column records of the column-coherent path, written lane-parallel and
   // read back per column at a wave-uniform address (an LDS broadcast, which
   // costs no vector-ALU slot, unlike a readlane): {alpha, beta} in float64
@@ -253,7 +230,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
   double2* crd = crd_s[threadIdx.x >> 6];
   float4* crf = crf_s[threadIdx.x >> 6];
 #pragma unroll
-  for (int t = 0; t < kJB; ++t)
+  for (int t = -kSpare; t < kJB + kSpare; ++t)
 #pragma unroll
     for (int cc = 0; cc < kJC; ++cc) g[t][cc][lane] = 0.f;
 
@@ -406,21 +383,43 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       }
     } } else if constexpr ((PATHS & kPathGeneral) != 0) {
       // pixels whose rays meet the support rectangle of this lane's voxels
-      // at this slice: bounding box of its projected corners
+      // at this slice: bounding box of its projected corners.  The ray
+      // through the source-relative point X meets the detector at
+      // col = (V x Q).X / N.X, row = (Q x U).X / N.X with N = U x V
+      // (q + col U + row V = lam X, lam = U.(V x Q) / N.X > 0 in front of
+      // the source): linear forms in X, whose a and b parts (the slice and
+      // the wave's b-run) are shared by the wave; each lane adds its c part.
+      const double nva = P.ub * P.vc - P.uc * P.vb;
+      const double nvb = P.uc * P.va - P.ua * P.vc;
+      const double nvc = P.ua * P.vb - P.ub * P.va;
+      const double mva = P.vb * P.qc - P.vc * P.qb;
+      const double mvb = P.vc * P.qa - P.va * P.qc;
+      const double mvc = P.va * P.qb - P.vb * P.qa;
+      const double rva = P.qb * P.uc - P.qc * P.ub;
+      const double rvb = P.qc * P.ua - P.qa * P.uc;
+      const double rvc = P.qa * P.ub - P.qb * P.ua;
+      const double lamn = P.ua * mva + P.ub * mvb + P.uc * mvc;
       double cmin = 1e300, cmax = -1e300, rmin = 1e300, rmax = -1e300;
       bool ok = true;
 #pragma unroll
-      for (int corner = 0; corner < 4; ++corner) {
-        const double xb = (corner & 1 ? jw + kJB + 1 + kEps : jw - kEps) -
-                          P.taub;
-        const double xc = (corner & 2 ? kd + (kJC + 1) + kEps : kd - kEps) -
-                          P.tauc;
-        double pc = 0.0, pr = 0.0;
-        ok = ok && project(P, arel, xb, xc, pc, pr);
-        cmin = fmin(cmin, pc);
-        cmax = fmax(cmax, pc);
-        rmin = fmin(rmin, pr);
-        rmax = fmax(rmax, pr);
+      for (int eb = 0; eb < 2; ++eb) {
+        const double xb = (eb ? jw + kJB + 1 + kEps : jw - kEps) - P.taub;
+        const double dn = fma(nvb, xb, nva * arel);
+        const double cn = fma(mvb, xb, mva * arel);
+        const double rn = fma(rvb, xb, rva * arel);
+#pragma unroll
+        for (int ec = 0; ec < 2; ++ec) {
+          const double xc =
+              (ec ? kd + (kJC + 1) + kEps : kd - kEps) - P.tauc;
+          const double inv = rcp64(fma(nvc, xc, dn));
+          const double pc = fma(mvc, xc, cn) * inv;
+          const double pr = fma(rvc, xc, rn) * inv;
+          ok = ok && isfinite(pc) && isfinite(pr) && lamn * inv > 0.0;
+          cmin = fmin(cmin, pc);
+          cmax = fmax(cmax, pc);
+          rmin = fmin(rmin, pr);
+          rmax = fmax(rmax, pr);
+        }
       }
       int32_t c_lo = 0, c_hi = A.cols - 1, r_lo = 0, r_hi = A.rows - 1;
       if (ok) {
@@ -445,7 +444,7 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       const double Da0 = fma(fr0, P.va, fma(fwc, P.ua, P.qa));
       const double Db0 = fma(fr0, P.vb, fma(fwc, P.ub, P.qb));
       const double Dc0 = fma(fr0, P.vc, fma(fwc, P.uc, P.qc));
-      const double inv0 = 1.0 / Da0;
+      const double inv0 = rcp64(Da0);
       const double tb = Db0 * inv0, tcv = Dc0 * inv0;
       const float ob =
           static_cast<float>(fma(tb, arel, P.taub) - (jw + kk::kHaloLo));
@@ -462,9 +461,10 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       // With u the row offset from the lane's first row and dc the column
       // offset from the wave's first column, dA = dc ua + u va (b, c
       // alike) and x_b = ob + arel (dB - tb dA) / (Da0 + dA): per column
-      // the numerator is affine in u with slope vb - tb va, and the
-      // denominator with slope va (c alike).
-      const float kvb = fmaf(-tbf, vaf, vbf), kvc = fmaf(-tcf, vaf, vcf);
+      // the numerator arel (dB - tb dA) is affine in u with slope
+      // arel (vb - tb va), and the denominator with slope va (c alike).
+      const float kvb = arf * fmaf(-tbf, vaf, vbf);
+      const float kvc = arf * fmaf(-tcf, vaf, vcf);
       const int32_t ncol = (anchor_ok && c_lo <= c_hi) ? c_hi - c_lo + 1 : 0;
       const int32_t wncol = kk_wave_max(ncol);
       // b-tap sums over a run of samples with the same tap index, added to
@@ -481,28 +481,39 @@ __global__ __launch_bounds__(kThreads, 6) void cone_fp_t_kernel(
       float S[kJC], T[kJC];
 #pragma unroll
       for (int cc = 0; cc < kJC; ++cc) S[cc] = T[cc] = 0.f;
+      // the workspace of this angle as bytes (a wave-uniform base) and a
+      // 32-bit byte offset per lane: the launcher keeps an angle's
+      // workspace below 4 GiB
+      const char* imgb = reinterpret_cast<const char*>(img);
+      const uint32_t nrow =
+          static_cast<uint32_t>(max(0, r_hi - r_lo + 1));
       for (int32_t j = 0; j < wncol; ++j) {
         const int32_t col = c_lo + j;  // this lane's column (in range if
                                        // j < ncol)
-        const float* ycol = img + col * cstride;
         const float dcf = static_cast<float>(col - wc_lo);
         const float dAc = dcf * uaf;
-        const float nb0 = fmaf(-tbf, dAc, dcf * ubf);
-        const float nc0 = fmaf(-tcf, dAc, dcf * ucf);
+        const float nb0 = arf * fmaf(-tbf, dAc, dcf * ubf);
+        const float nc0 = arf * fmaf(-tcf, dAc, dcf * ucf);
         const float den0 = da0 + dAc;
-        const int32_t rl = r_lo;
-        const int32_t rh = j < ncol ? r_hi : r_lo - 1;
+        uint32_t o = 4u * (static_cast<uint32_t>(col) *
+                               static_cast<uint32_t>(cstride) +
+                           static_cast<uint32_t>(r_lo));
+        const uint32_t oend = o + 4u * (j < ncol ? nrow : 0u);
         float drf = 0.f;
-        for (int32_t r = rl; r <= rh; ++r, drf += 1.f) {
-          const float sc = arf * __builtin_amdgcn_rcpf(fmaf(drf, vaf, den0));
-          const float xc = fmaf(fmaf(drf, kvc, nc0), sc, oc);
-          const float xb = fmaf(fmaf(drf, kvb, nb0), sc, ob);
+        for (; o < oend; o += 4u, drf += 1.f) {
+          // the load first: its latency overlaps the coordinates below.
+          // The empty asm keeps the offset a 32-bit register (an SGPR base
+          // plus a VGPR offset), not a 64-bit pointer stepped per row.
+          asm volatile("" : "+v"(o));
+          const float yv = *reinterpret_cast<const float*>(imgb + o);
+          const float rc = __builtin_amdgcn_rcpf(fmaf(drf, vaf, den0));
+          const float xc = fmaf(fmaf(drf, kvc, nc0), rc, oc);
+          const float xb = fmaf(fmaf(drf, kvb, nb0), rc, ob);
           const float fl = floorf(xb);
           if (fl != tcur) {
             close_run(g, lane, tcur, S, T);
             tcur = fl;
           }
-          const float yv = ycol[r];
           const float yfb = yv * (xb - fl);
           const float w0 = fmaxf(0.f, 1.f - fabsf(xc));
           const float w1 = __builtin_amdgcn_fmed3f(0.f, xc, 2.f - xc);
@@ -558,6 +569,12 @@ int kk::fp_transpose_launch(const kk_plan_s* plan, const float* proj,
   if (int64_t(rows) * cols >= (int64_t(1) << 31))
     return kk::fail(KK_ERR_UNSUPPORTED,
                     "kk_cone_fp_transpose: detector of 2^31 pixels or more");
+  // the gather addresses one angle's padded workspace with 32-bit byte
+  // offsets
+  if (int64_t(cols) * (rows + 2 * kRowPad) * 4 >= (int64_t(1) << 32))
+    return kk::fail(KK_ERR_UNSUPPORTED,
+                    "kk_cone_fp_transpose: detector of 2^30 padded pixels "
+                    "or more");
   for (int t = 0; t < 3; ++t)
     if (vol_shape[t] != plan->fp_shape[t])
       return kk::fail(KK_ERR_SHAPE,

@@ -314,24 +314,27 @@ _KEPS = 1e-6    # its widening of candidate intervals (voxels)
 
 
 def _project_corner(P, xa, xb, xc):
-    """cone_fp_t_kernel's project(): detector (col, row) of the ray through
-    the source-relative point X (float64, vectorised); ok iff finite and in
-    front of the source."""
+    """cone_fp_t_kernel's box corners: detector (col, row) of the ray
+    through the source-relative point X (float64, vectorised) as the linear
+    forms col = (V x Q).X / N.X, row = (Q x U).X / N.X with N = U x V; ok
+    iff finite and in front of the source (U.(V x Q) / N.X > 0)."""
     g = {k: float(P[k]) for k in ("qa", "qb", "qc", "ua", "ub", "uc",
                                   "va", "vb", "vc")}
-    vx = (g["vb"] * xc - g["vc"] * xb, g["vc"] * xa - g["va"] * xc,
-          g["va"] * xb - g["vb"] * xa)
-    qx = (g["qb"] * xc - g["qc"] * xb, g["qc"] * xa - g["qa"] * xc,
-          g["qa"] * xb - g["qb"] * xa)
-    vq = (g["vb"] * g["qc"] - g["vc"] * g["qb"],
-          g["vc"] * g["qa"] - g["va"] * g["qc"],
-          g["va"] * g["qb"] - g["vb"] * g["qa"])
+    n = (g["ub"] * g["vc"] - g["uc"] * g["vb"],
+         g["uc"] * g["va"] - g["ua"] * g["vc"],
+         g["ua"] * g["vb"] - g["ub"] * g["va"])
+    m = (g["vb"] * g["qc"] - g["vc"] * g["qb"],
+         g["vc"] * g["qa"] - g["va"] * g["qc"],
+         g["va"] * g["qb"] - g["vb"] * g["qa"])
+    r = (g["qb"] * g["uc"] - g["qc"] * g["ub"],
+         g["qc"] * g["ua"] - g["qa"] * g["uc"],
+         g["qa"] * g["ub"] - g["qb"] * g["ua"])
+    lamn = g["ua"] * m[0] + g["ub"] * m[1] + g["uc"] * m[2]
     with np.errstate(divide="ignore", invalid="ignore"):
-        den = g["ua"] * vx[0] + g["ub"] * vx[1] + g["uc"] * vx[2]
-        col = -(g["qa"] * vx[0] + g["qb"] * vx[1] + g["qc"] * vx[2]) / den
-        row = -(g["ua"] * qx[0] + g["ub"] * qx[1] + g["uc"] * qx[2]) / den
-        lam = (g["ua"] * vq[0] + g["ub"] * vq[1] + g["uc"] * vq[2]) / den
-    return col, row, np.isfinite(col) & np.isfinite(row) & (lam > 0)
+        inv = 1.0 / (n[0] * xa + n[1] * xb + n[2] * xc)
+        col = (m[0] * xa + m[1] * xb + m[2] * xc) * inv
+        row = (r[0] * xa + r[1] * xb + r[2] * xc) * inv
+    return col, row, np.isfinite(col) & np.isfinite(row) & (lamn * inv > 0)
 
 
 def _int_range(lo, hi, n):
@@ -526,21 +529,24 @@ def _fpt_kernel_weights(P, i, R, C, J, K, rows, cols, nb):
         oc = (tc * arel + g["tauc"] - (kl + lo)).astype(f32)
         F = {x: f32(g[x]) for x in ("ua", "ub", "uc", "va", "vb", "vc")}
         tbf, tcf = tb.astype(f32), tc.astype(f32)
+        arf = f32(arel)
         dcf = (C - wc_lo).astype(f32)
         drf = (R - r_lo).astype(f32)
-        # per column: numerators affine in the row offset, slopes
-        # vb - tb va and vc - tc va; denominator Da0 + dc ua + u va
+        # per column: numerators arel (dB - tb dA) affine in the row
+        # offset, slopes arel (vb - tb va) and arel (vc - tc va);
+        # denominator Da0 + dc ua + u va
         dAc = (dcf * F["ua"]).astype(f32)
-        nb0 = _f32fma(-tbf, dAc, (dcf * F["ub"]).astype(f32))
-        nc0 = _f32fma(-tcf, dAc, (dcf * F["uc"]).astype(f32))
+        nb0 = (arf * _f32fma(-tbf, dAc, (dcf * F["ub"]).astype(f32))
+               ).astype(f32)
+        nc0 = (arf * _f32fma(-tcf, dAc, (dcf * F["uc"]).astype(f32))
+               ).astype(f32)
         den0 = (Da0.astype(f32) + dAc).astype(f32)
-        kvb = _f32fma(-tbf, F["va"], F["vb"])
-        kvc = _f32fma(-tcf, F["va"], F["vc"])
+        kvb = (arf * _f32fma(-tbf, F["va"], F["vb"])).astype(f32)
+        kvc = (arf * _f32fma(-tcf, F["va"], F["vc"])).astype(f32)
         den = _f32fma(drf, F["va"], den0)
         rcp = (1.0 / den.astype(np.float64)).astype(f32)
-        sc = (f32(arel) * rcp).astype(f32)
-        xb = _f32fma(_f32fma(drf, kvb, nb0), sc, ob)
-        xc = _f32fma(_f32fma(drf, kvc, nc0), sc, oc)
+        xb = _f32fma(_f32fma(drf, kvb, nb0), rcp, ob)
+        xc = _f32fma(_f32fma(drf, kvc, nc0), rcp, oc)
         w0 = np.maximum(f32(0), (f32(1) - np.abs(xc)).astype(f32))
         w1 = np.clip(np.minimum(xc, (f32(2) - xc).astype(f32)), f32(0), None)
         wc = np.where(K - kl == 0, w0, w1).astype(f32)
