@@ -42,6 +42,8 @@
 // Candidate intervals are widened by 1e-6 voxel in coordinate space, far
 // above the float64 rounding of the sample coordinates; samples outside
 // the support get zero weight.
+#include <algorithm>
+
 #include "kk_internal.h"
 
 namespace {
@@ -64,6 +66,16 @@ struct FptArgs {
   int32_t n_angles;
   int32_t mode;           // 0: vol = A^T y, 1: vol += A^T y
 };
+
+// Angles per cone_fp_t_kernel launch.  A workgroup walks every angle of
+// its launch, so the resident workgroups (which start and retire at
+// different times) are spread over all of them; neighbouring slices read
+// nearly the same pixels of an angle, but share them in the caches only
+// if few angles are in flight.  A launch takes the angles whose workspace
+// fits this budget (three quarters of MI355X's 256 MiB last-level cache;
+// each launch after the first adds to the volume, one read-modify-write
+// of it per launch).
+constexpr int64_t kChunkBytes = int64_t(192) << 20;
 
 constexpr int kTile = 64;  // ray_weight_kernel: 64 x 64 pixels per block
 // Zero rows before and after every column of the workspace: a lane's row
@@ -592,6 +604,9 @@ int kk::fp_transpose_launch(const kk_plan_s* plan, const float* proj,
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return kk::hip_fail(e, "ray_weight_kernel");
   }
+  const int64_t angle_bytes = int64_t(cols) * (rows + 2 * kRowPad) * 4;
+  const int32_t chunk = static_cast<int32_t>(std::min<int64_t>(
+      INT32_MAX, std::max<int64_t>(1, kChunkBytes / angle_bytes)));
   bool first = true;
   for (int a = 0; a < 3; ++a) {
     const int32_t n = plan->fr_start[a + 1] - plan->fr_start[a];
@@ -613,22 +628,24 @@ int kk::fp_transpose_launch(const kk_plan_s* plan, const float* proj,
     const int64_t blocks = int64_t(A.na) * A.tiles_b * A.tiles_c;
     if (blocks >= (int64_t(1) << 31))
       return kk::fail(KK_ERR_UNSUPPORTED, "kk_cone_fp_transpose: grid");
-    A.angles = plan->fr_idx + plan->fr_start[a];
-    A.n_angles = n;
-    A.mode = (first && !additive) ? 0 : 1;
-    first = false;
     const dim3 grid(static_cast<unsigned>(blocks));
-    if (plan->fp_all_colc)
-      hipLaunchKernelGGL(cone_fp_t_kernel<kPathColumn>, grid,
-                         dim3(kThreads), 0, st, plan->fp, A);
-    else if (!plan->fp_any_colc)
-      hipLaunchKernelGGL(cone_fp_t_kernel<kPathGeneral>, grid,
-                         dim3(kThreads), 0, st, plan->fp, A);
-    else
-      hipLaunchKernelGGL((cone_fp_t_kernel<kPathColumn | kPathGeneral>),
-                         grid, dim3(kThreads), 0, st, plan->fp, A);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_t_kernel");
+    for (int32_t q0 = 0; q0 < n; q0 += chunk) {
+      A.angles = plan->fr_idx + plan->fr_start[a] + q0;
+      A.n_angles = std::min(chunk, n - q0);
+      A.mode = (first && !additive) ? 0 : 1;
+      first = false;
+      if (plan->fp_all_colc)
+        hipLaunchKernelGGL(cone_fp_t_kernel<kPathColumn>, grid,
+                           dim3(kThreads), 0, st, plan->fp, A);
+      else if (!plan->fp_any_colc)
+        hipLaunchKernelGGL(cone_fp_t_kernel<kPathGeneral>, grid,
+                           dim3(kThreads), 0, st, plan->fp, A);
+      else
+        hipLaunchKernelGGL((cone_fp_t_kernel<kPathColumn | kPathGeneral>),
+                           grid, dim3(kThreads), 0, st, plan->fp, A);
+      hipError_t e = hipGetLastError();
+      if (e != hipSuccess) return kk::hip_fail(e, "cone_fp_t_kernel");
+    }
   }
   return KK_OK;
 }

@@ -766,6 +766,43 @@ def autograd_diff(ceiling=None):
     return res
 
 
+def large_transpose(reps=2):
+    """The exact transpose against the forward projection at the largest
+    workload (1024^3 volume, 1440 angles with seeded +-1 degree wobble and
+    +-3 pixel drift, 1536^2 detector): the general-geometry path of the
+    transpose, HIP-event times on the launching stream after a warm-up
+    call of each."""
+    import numpy as np
+    import torch
+
+    import kernelkit as kk
+    from kernelkit import workloads
+
+    dev = torch.device("cuda", 0)
+    pgs, vg = workloads.config_geometry("large")
+    x = workloads.ellipsoid_phantom_device(vg.shape, n=10, seed=0, device=dev)
+    A = kk.XrayTransform(pgs, vg)
+    stream = torch.cuda.current_stream(dev)
+    y = A(x)
+    t_fp = _event_ms(lambda: A(x, out=y), stream, reps=reps)
+    _log(f"large: forward projection {t_fp:.1f} ms")
+    v = torch.empty_like(x)
+    A.matched_T(y, out=v)
+    torch.cuda.synchronize()
+    _log("large: transpose warm-up done")
+    t_t = _event_ms(lambda: A.matched_T(y, out=v), stream, reps=reps)
+    _log(f"large: transpose {t_t:.1f} ms")
+    updates = int(np.prod(vg.shape)) * len(pgs)
+    return {
+        "workload": ("1024^3 volume, 1440 angles (seeded +-1 deg wobble, "
+                     "+-3 px drift), 1536^2 detector"),
+        "fp_ms": round(t_fp, 1),
+        "transpose_ms": round(t_t, 1),
+        "transpose_vs_fp": round(t_t / t_fp, 3),
+        "transpose_gups": round(updates / (t_t * 1e-3) / 1e9, 1),
+    }
+
+
 def large_unrolled(iters=5):
     """Differentiation through ``iters`` unrolled SIRT iterations at the
     largest workload (1024^3 volume, 1440 angles with seeded +-1 degree
@@ -865,9 +902,9 @@ def main(argv=None):
     ap.add_argument("--extras", default="batched,autograd,algorithms",
                     help="comma-separated extra measurements after the "
                          "FP/BP step at the full config: batched, "
-                         "autograd, algorithms, large, cpu_small (the "
-                         "last two not in the default list: several "
-                         "minutes each)")
+                         "autograd, algorithms, large, large_transpose, "
+                         "cpu_small (the last three not in the default "
+                         "list: a minute or more each)")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     metavar="MIB",
                     help="after the timed steps, run one plain device copy "
@@ -1042,7 +1079,7 @@ def main(argv=None):
     extras = set() if args.quick or args.config != "full" else {
         e.strip() for e in args.extras.split(",") if e.strip()}
     unknown = extras - {"batched", "autograd", "algorithms", "large",
-                        "cpu_small"}
+                        "large_transpose", "cpu_small"}
     if unknown:
         print(f"unknown extras: {sorted(unknown)}", file=sys.stderr)
         return 2
@@ -1058,10 +1095,15 @@ def main(argv=None):
     if "algorithms" in extras:
         _log("FDK and SIRT at full size ...")
         out["algorithms"] = algorithms_full(pgs, vg, x)
-    if "large" in extras:
-        _log("unrolled differentiation at 1024^3 ...")
+    if "large" in extras or "large_transpose" in extras:
         del fptor, bptor, y, v
         torch.cuda.empty_cache()
+    if "large_transpose" in extras:
+        _log("transpose vs forward projection at 1024^3 ...")
+        out["large_transpose"] = large_transpose()
+        torch.cuda.empty_cache()
+    if "large" in extras:
+        _log("unrolled differentiation at 1024^3 ...")
         out["large"] = large_unrolled()
     if not args.no_cpu_baseline:
         _log("cpu baseline (FP64 oracle, 1 core and 16 processes) ...")
