@@ -127,14 +127,23 @@ __global__ __launch_bounds__(256) void ray_weight_kernel(
   }
 }
 
-// 1 / d in float64: the hardware estimate and two Newton steps (a few
-// units in the last place; non-finite for d = 0).
+// 1 / d in float64 for the detector boxes and the walk's anchor: the
+// hardware estimate and one Newton step, which squares its relative error
+// (to ~1e-14 from the estimate's ~1e-7: far below the 1e-6 voxel widening
+// of the boxes and the float32 rounding of the anchor offsets); non-finite
+// for d = 0.
 __device__ __forceinline__ double rcp64(double d) {
-  double r = __builtin_amdgcn_rcp(d);
-  double e = fma(-d, r, 1.0);
-  r = fma(r, e, r);
-  e = fma(-d, r, 1.0);
-  return fma(r, e, r);
+  const double r = __builtin_amdgcn_rcp(d);
+  return fma(r, fma(-d, r, 1.0), r);
+}
+
+// Least / greatest value over a wavefront whose 64 lanes are all active
+// (DPP steps and a readlane, no LDS round trips), as a scalar.
+__device__ __forceinline__ int32_t wave_min_all(int32_t v) {
+  return __builtin_amdgcn_readfirstlane(__reduce_min_sync(~0ull, v));
+}
+__device__ __forceinline__ int32_t wave_max_all(int32_t v) {
+  return __builtin_amdgcn_readfirstlane(__reduce_max_sync(~0ull, v));
 }
 
 // Integer interval [ceil(lo), floor(hi)] clipped to [0, n - 1].
@@ -450,7 +459,7 @@ __global__ __launch_bounds__(kThreads, PATHS == 3 ? 5 : 6) void cone_fp_t_kernel
       // quantity of the box's size, so its float32 rounding is ~1e-6
       // voxel.  Weights are the tents 1 - |x - centre| of the forward
       // kernel's bilinear taps.
-      const int32_t wc_lo = kk_wave_min(c_lo <= c_hi ? c_lo : A.cols);
+      const int32_t wc_lo = wave_min_all(c_lo <= c_hi ? c_lo : A.cols);
       const double fwc = static_cast<double>(wc_lo);
       const double fr0 = static_cast<double>(r_lo);
       const double Da0 = fma(fr0, P.va, fma(fwc, P.ua, P.qa));
@@ -478,7 +487,7 @@ __global__ __launch_bounds__(kThreads, PATHS == 3 ? 5 : 6) void cone_fp_t_kernel
       const float kvb = arf * fmaf(-tbf, vaf, vbf);
       const float kvc = arf * fmaf(-tcf, vaf, vcf);
       const int32_t ncol = (anchor_ok && c_lo <= c_hi) ? c_hi - c_lo + 1 : 0;
-      const int32_t wncol = kk_wave_max(ncol);
+      const int32_t wncol = wave_max_all(ncol);
       // b-tap sums over a run of samples with the same tap index, added to
       // the lane's LDS words when the index changes: S = sum of y w_c,
       // T = sum of y w_c fb, so the taps t and t + 1 receive S - T and T.
