@@ -19,7 +19,9 @@
   iterations replayed from a HIP graph equal the eager ones and the
   per-problem ``kk.sirt``.
 * large: 1024^3, 1536^2 detector, wobbling orbit with detector drift: the
-  exact transpose on voxel blocks against the oracle's transpose.
+  exact transpose on voxel blocks against the oracle's transpose, and the
+  transpose over 45 angles (launches of 20 angles) against the sum over
+  three 15-angle subsets (linearity, relative L2 <= 1e-6).
 
 Bar: relative L2 <= 1e-5 and pointwise <= 1e-4 max|ref| (``assert_parity``);
 bit equality where the same kernels run on the same data.
@@ -344,6 +346,38 @@ def test_large_transpose_blocks(device):
         assert float(np.abs(ref).max()) > 0
         assert_parity(got, ref, label=f"large transpose {lo}")
     del t
+    torch.cuda.empty_cache()
+
+
+def test_large_transpose_angle_chunks_are_linear(device):
+    """At 1536^2 an angle's transpose workspace is 9.6 MB, so a launch
+    takes 20 angles: 45 wobbling angles run in several launches per ray
+    axis, the later ones adding to the volume.  Their transpose equals the
+    sum of the transposes of three interleaved 15-angle subsets (one
+    launch per axis each), up to float32 summation order."""
+    torch = _torch()
+    pgs, vg = workloads.config_geometry("large")
+    sub = [pgs[k] for k in range(0, 1440, 32)]
+    assert len(sub) == 45
+    A = kk.XrayTransform(sub, vg)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    y = torch.rand(A.range_shape, generator=g, device="cuda")
+    _say("large: transpose of 45 wobbling angles (chunked launches)")
+    t = A.matched_T(y)
+    acc = torch.zeros_like(t)
+    for part in range(3):
+        ks = list(range(part, len(sub), 3))
+        _say(f"large: transpose of the angle subset {part}")
+        Ak = kk.XrayTransform([sub[k] for k in ks], vg)
+        acc += Ak.matched_T(y[ks].contiguous())
+        del Ak
+    torch.cuda.synchronize()
+    d = (t - acc).double()
+    rel = float(d.norm() / acc.double().norm())
+    point = float(d.abs().max() / acc.abs().max())
+    assert float(acc.abs().max()) > 0
+    assert rel < 1e-6 and point < 1e-5, (rel, point)
+    del t, acc, d
     torch.cuda.empty_cache()
 
 
