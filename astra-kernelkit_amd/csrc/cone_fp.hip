@@ -37,12 +37,15 @@
 //   b-coordinate, its tap row and weight are shared by the thread's R rows
 //   ("column-coherent" angles, flagged per angle on the host);
 // * per-angle parameters are wave-uniform (scalar loads); workgroups are
-//   ordered row band -> column tile -> angle, and runs of kXcdRun
-//   consecutive ids (consecutive angles of one column tile) go to one XCD,
-//   which holds about that many such workgroups at a time: neighbouring
-//   angles read nearly the same texture lines, so its L2 serves them
-//   (measured on MI355X at 512^3 / 720 angles: 90 ms, against 110 ms with
-//   column tiles fastest and workgroups dealt round-robin to the XCDs).
+//   ordered in groups of kGroupAngles consecutive angles x kGroupCols
+//   neighbouring column tiles (angle fastest), groups ordered angle group
+//   -> column-tile group -> row band, and each group's ids go to one XCD:
+//   neighbouring angles and columns read nearly the same texture lines, so
+//   its L2 serves them (measured on MI355X at 512^3 / 720 angles: 86.9 ms
+//   with 16 x 8, 91.0 ms with runs of 64 angles of one column tile, 110 ms
+//   with column tiles fastest and workgroups dealt round-robin to the
+//   XCDs; 16 x 4, 32 x 4 and 16 x 16 took 87.7, 88.1 and 90.5 ms).
+#include <algorithm>
 #include <new>
 #include <vector>
 
@@ -58,9 +61,13 @@ constexpr int kWave = 64;
 // 4 columns).  Batches of small detectors: 4.
 constexpr int kColsSingle = 16;
 constexpr int kColsBatch = 4;
-// Consecutive logical workgroup ids given to one XCD: 32 CUs per XCD x 2
-// resident workgroups of 1024 threads.
+// Consecutive logical workgroup ids given to one XCD in batches: 32 CUs
+// per XCD x 2 resident workgroups of 1024 threads.
 constexpr int32_t kXcdRun = 64;
+// Single problems: the workgroups of kGroupAngles consecutive angles x
+// kGroupCols neighbouring column tiles go to one XCD together.
+constexpr int32_t kGroupAngles = 16;
+constexpr int32_t kGroupCols = 8;
 
 struct FpArgs {
   const char* tex;  // frame textures (byte base)
@@ -79,6 +86,9 @@ struct FpArgs {
   int32_t row_tiles, col_tiles;
   int32_t mode;  // 0: out = p, 1: out += p, 2: out = (p - y) * R
   int32_t xcd_group;
+  // workgroup groups: grp_a angles x grp_c column tiles (ids past the
+  // angles or column tiles, which pad the grid to whole groups, return)
+  int32_t grp_a, grp_c;
   int64_t total_blocks;
 };
 
@@ -364,8 +374,9 @@ __device__ __forceinline__ void fp_tile_at(
   }
 }
 
-// Decode logical block id `bid` (angle fastest, then column tile, then row
-// band) and trace that tile.  TABLE_IN_MEMORY: the angle table pointer was
+// Decode logical block id `bid` (groups of grp_a angles x grp_c column
+// tiles, angle fastest inside a group; groups ordered angle group, then
+// column-tile group, then row band) and trace that tile.  TABLE_IN_MEMORY: the angle table pointer was
 // read from a descriptor (batch kernels); the record is then copied through
 // constant-memory (scalar) loads first.  A table passed as a kernel
 // argument is read in place (the compiler scalarizes it by itself; the
@@ -374,10 +385,18 @@ template <int R, bool ALL_COLC, bool PIPE, int COLS, bool TABLE_IN_MEMORY,
           bool PTRS = false>
 __device__ __forceinline__ void fp_tile(const kk::FpAngle* __restrict__ tab,
                                         const FpArgs& A, int64_t bid) {
-  const int32_t ang = static_cast<int32_t>(bid % A.n_angles);
-  const int64_t t = bid / A.n_angles;
-  const int32_t ct = static_cast<int32_t>(t % A.col_tiles);
-  const int32_t rt = static_cast<int32_t>(t / A.col_tiles);
+  const int32_t ga = A.grp_a, gc = A.grp_c;
+  const int64_t inner = bid % (int64_t(ga) * gc);
+  const int64_t outer = bid / (int64_t(ga) * gc);
+  const int32_t nag = (A.n_angles + ga - 1) / ga;
+  const int32_t ncg = (A.col_tiles + gc - 1) / gc;
+  const int32_t ang = static_cast<int32_t>(outer % nag) * ga +
+                      static_cast<int32_t>(inner % ga);
+  const int64_t t = outer / nag;
+  const int32_t ct = static_cast<int32_t>(t % ncg) * gc +
+                     static_cast<int32_t>(inner / ga);
+  const int32_t rt = static_cast<int32_t>(t / ncg);
+  if (ang >= A.n_angles || ct >= A.col_tiles) return;  // grid padding
   if constexpr (TABLE_IN_MEMORY) {
     const kk::FpAngle rec = kk_uniform_load(tab + ang);
     fp_tile_at<R, ALL_COLC, PIPE, COLS, PTRS>(&rec, 0, A, ang, ct, rt);
@@ -467,10 +486,21 @@ int make_fp_args(const kk::FpSpec& S, int32_t rpt, int32_t cols_per_block,
   A.row_tiles = (S.rows + kWave * rpt - 1) / (kWave * rpt);
   A.col_tiles = (S.cols + cols_per_block - 1) / cols_per_block;
   A.mode = S.mode;
-  A.total_blocks = int64_t(A.row_tiles) * A.n_angles * A.col_tiles;
+  if (cols_per_block == kColsSingle) {
+    A.grp_a = std::min(kGroupAngles, A.n_angles);
+    A.grp_c = std::min(kGroupCols, A.col_tiles);
+    A.xcd_group = A.grp_a * A.grp_c;
+  } else {
+    // batches: all angles of one column tile, then the next column tile
+    A.grp_a = A.n_angles;
+    A.grp_c = 1;
+    A.xcd_group = kXcdRun;
+  }
+  A.total_blocks = int64_t(A.row_tiles) *
+                   ((A.n_angles + A.grp_a - 1) / A.grp_a) * A.grp_a *
+                   ((A.col_tiles + A.grp_c - 1) / A.grp_c) * A.grp_c;
   if (A.total_blocks >= (int64_t(1) << 31))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_fp: grid too large");
-  A.xcd_group = kXcdRun;
   return KK_OK;
 }
 

@@ -23,7 +23,13 @@ The JSON line also carries:
   WRITE_SIZE counters (scaled by a plain-copy calibration), read from
   ``profiles/cone_fp_bp_512cube_720_traffic.json`` when present, and
   ``traffic_GBs`` / ``traffic_frac_of_hbm`` that amount at this run's
-  kernel time against the 8 TB/s HBM peak;
+  kernel time against the 8 TB/s HBM peak.  The samples are the
+  algorithm's (every ray samples every slice of its major axis, zeros
+  outside the volume included); the kernel skips the samples outside its
+  wavefronts' ray supports, so ``frac`` can exceed 1.  ``issued_bytes``
+  (16 B per gather the kernel issues, from the SQ_INSTS_VMEM_RD counter in
+  ``profiles/cone_fp_bp_512cube_720_gathers.json``) and ``issued_frac``
+  give the rate of the gathers it does issue against the same ceiling;
 * ``cpu_baseline``: the FP64 NumPy oracle in 16 host processes of one core
   each (``single_core``: one process), timed on a bounded sample of the
   same workload (one angle, a band of detector rows for FP and a slab of
@@ -51,6 +57,8 @@ L1_RETURN_GBS = 256 * 64 * 2.4
 HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = os.path.join(ROOT, "profiles",
                             "cone_fp_bp_512cube_720_traffic.json")
+GATHERS_FILE = os.path.join(ROOT, "profiles",
+                            "cone_fp_bp_512cube_720_gathers.json")
 
 
 def _log(msg):
@@ -1006,10 +1014,13 @@ def main(argv=None):
     fp_bytes = 16 * samples
     bp_bytes = 16 * updates
     achieved = fp_bytes / (t_fp * 1e-3) / 1e9
-    traffic = None
+    traffic = issued = None
     if os.path.exists(TRAFFIC_FILE) and args.config == "full":
         with open(TRAFFIC_FILE) as f:
             traffic = json.load(f).get("cone_fp_kernel_bytes")
+    if os.path.exists(GATHERS_FILE) and args.config == "full":
+        with open(GATHERS_FILE) as f:
+            issued = json.load(f).get("cone_fp_kernel_issued_bytes")
     compulsory_fp = 4 * (int(np.prod(shape)) + n * rows * cols)
     out = {
         "metric": METRIC,
@@ -1050,6 +1061,13 @@ def main(argv=None):
             "unit": "GB/s",
             "frac": round(achieved / ceiling, 4),
             "frac_of_best_probe": round(achieved / probe_best, 4),
+            # the gathers the kernel issues (it skips samples outside its
+            # wavefronts' ray supports, which the algorithm counts)
+            "issued_bytes": issued,
+            "issued_frac": (None if issued is None else round(
+                issued / (t_fp * 1e-3) / 1e9 / ceiling, 4)),
+            "issued_over_algorithmic": (
+                None if issued is None else round(issued / fp_bytes, 4)),
             "traffic": traffic,
             "algorithmic_bytes": fp_bytes,
             "bytes_per_update": round(fp_bytes / updates, 4),

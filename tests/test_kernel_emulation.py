@@ -796,20 +796,40 @@ def test_xcd_remap_is_a_bijection_with_runs_per_xcd(total, group):
             assert np.array_equal(ids, c0 + x * group + np.arange(group))
 
 
-def test_fp_tile_decode_covers_every_tile_once():
-    """The forward kernel's logical id -> (angle, column tile, row band)
-    decode (angle fastest) composed with the XCD remap visits every tile
-    once, for the 512^3 / 720-angle launch shape (16-column workgroups,
-    128-row bands)."""
-    n_ang, col_tiles, row_tiles = 720, 48, 6
-    total = n_ang * col_tiles * row_tiles
-    bid = _xcd_remap(np.arange(total, dtype=np.int64), total, 64)
-    ang = bid % n_ang
-    t = bid // n_ang
-    ct, rt = t % col_tiles, t // col_tiles
-    key = (rt * col_tiles + ct) * n_ang + ang
-    assert np.array_equal(np.sort(key), np.arange(total))
-    assert rt.max() == row_tiles - 1 and ct.max() == col_tiles - 1
+def _fp_tile_decode(bid, n_ang, col_tiles, ga, gc):
+    """fp_tile's logical id -> (angle, column tile, row band): groups of
+    ga angles x gc column tiles (angle fastest inside), groups ordered
+    angle group -> column-tile group -> row band; ids past the angles or
+    the column tiles pad the grid to whole groups."""
+    inner, outer = bid % (ga * gc), bid // (ga * gc)
+    nag = -(-n_ang // ga)
+    ncg = -(-col_tiles // gc)
+    ang = (outer % nag) * ga + inner % ga
+    t = outer // nag
+    ct = (t % ncg) * gc + inner // ga
+    return ang, ct, t // ncg
+
+
+@pytest.mark.parametrize("n_ang,col_tiles,row_tiles,ga,gc", [
+    (720, 48, 6, 16, 8),      # 512^3 / 720 angles, 768^2: no padding
+    (360, 24, 3, 16, 8),      # 256^3 / 360, 384^2
+    (45, 10, 2, 16, 8),       # padding along angles and column tiles
+    (7, 3, 1, 7, 3),          # groups clipped to the problem
+    (90, 5, 2, 90, 1)])       # batch order: all angles of a column tile
+def test_fp_tile_decode_covers_every_tile_once(n_ang, col_tiles, row_tiles,
+                                               ga, gc):
+    """The forward kernel's workgroup decode composed with the XCD remap
+    (groups of ga x gc ids per XCD) visits every (angle, column tile, row
+    band) once; the padding ids fall outside."""
+    total = (row_tiles * (-(-n_ang // ga)) * ga *
+             (-(-col_tiles // gc)) * gc)
+    bid = _xcd_remap(np.arange(total, dtype=np.int64), total, ga * gc)
+    ang, ct, rt = _fp_tile_decode(bid, n_ang, col_tiles, ga, gc)
+    live = (ang < n_ang) & (ct < col_tiles)
+    assert rt.max() == row_tiles - 1
+    key = (rt[live] * col_tiles + ct[live]) * n_ang + ang[live]
+    assert np.array_equal(np.sort(key),
+                          np.arange(n_ang * col_tiles * row_tiles))
 
 
 def _uniform_i2d_bits(i):
