@@ -44,7 +44,9 @@
 //   its L2 serves them (measured on MI355X at 512^3 / 720 angles: 86.9 ms
 //   with 16 x 8, 91.0 ms with runs of 64 angles of one column tile, 110 ms
 //   with column tiles fastest and workgroups dealt round-robin to the
-//   XCDs; 16 x 4, 32 x 4 and 16 x 16 took 87.7, 88.1 and 90.5 ms).
+//   XCDs; 16 x 4, 32 x 4 and 16 x 16 took 87.7, 88.1 and 90.5 ms).  In
+//   batches (256-thread workgroups) groups of 16 x 16 took the batched
+//   forward projection of the 64-problem workload from 104.8 to 98.1 ms.
 #include <algorithm>
 #include <new>
 #include <vector>
@@ -61,13 +63,14 @@ constexpr int kWave = 64;
 // 4 columns).  Batches of small detectors: 4.
 constexpr int kColsSingle = 16;
 constexpr int kColsBatch = 4;
-// Consecutive logical workgroup ids given to one XCD in batches: 32 CUs
-// per XCD x 2 resident workgroups of 1024 threads.
-constexpr int32_t kXcdRun = 64;
-// Single problems: the workgroups of kGroupAngles consecutive angles x
-// kGroupCols neighbouring column tiles go to one XCD together.
+// The workgroups of kGroupAngles consecutive angles x kGroupCols (single
+// problems; kGroupColsBatch in batches) neighbouring column tiles go to
+// one XCD together.  An XCD holds 64 single-problem workgroups (32 CUs x
+// 2 of 1024 threads) or 256 batch workgroups (32 x 8 of 256 threads);
+// measured on MI355X: 16 x 8 and 16 x 16 were the fastest groups.
 constexpr int32_t kGroupAngles = 16;
 constexpr int32_t kGroupCols = 8;
+constexpr int32_t kGroupColsBatch = 16;
 
 struct FpArgs {
   const char* tex;  // frame textures (byte base)
@@ -486,16 +489,11 @@ int make_fp_args(const kk::FpSpec& S, int32_t rpt, int32_t cols_per_block,
   A.row_tiles = (S.rows + kWave * rpt - 1) / (kWave * rpt);
   A.col_tiles = (S.cols + cols_per_block - 1) / cols_per_block;
   A.mode = S.mode;
-  if (cols_per_block == kColsSingle) {
-    A.grp_a = std::min(kGroupAngles, A.n_angles);
-    A.grp_c = std::min(kGroupCols, A.col_tiles);
-    A.xcd_group = A.grp_a * A.grp_c;
-  } else {
-    // batches: all angles of one column tile, then the next column tile
-    A.grp_a = A.n_angles;
-    A.grp_c = 1;
-    A.xcd_group = kXcdRun;
-  }
+  A.grp_a = std::min(kGroupAngles, A.n_angles);
+  A.grp_c = std::min(
+      cols_per_block == kColsSingle ? kGroupCols : kGroupColsBatch,
+      A.col_tiles);
+  A.xcd_group = A.grp_a * A.grp_c;
   A.total_blocks = int64_t(A.row_tiles) *
                    ((A.n_angles + A.grp_a - 1) / A.grp_a) * A.grp_a *
                    ((A.col_tiles + A.grp_c - 1) / A.grp_c) * A.grp_c;

@@ -815,7 +815,8 @@ def _fp_tile_decode(bid, n_ang, col_tiles, ga, gc):
     (360, 24, 3, 16, 8),      # 256^3 / 360, 384^2
     (45, 10, 2, 16, 8),       # padding along angles and column tiles
     (7, 3, 1, 7, 3),          # groups clipped to the problem
-    (90, 5, 2, 90, 1)])       # batch order: all angles of a column tile
+    (90, 5, 2, 16, 5),        # a batch problem with 5 column tiles
+    (90, 40, 2, 16, 16)])     # batch groups
 def test_fp_tile_decode_covers_every_tile_once(n_ang, col_tiles, row_tiles,
                                                ga, gc):
     """The forward kernel's workgroup decode composed with the XCD remap
