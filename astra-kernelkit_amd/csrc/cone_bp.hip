@@ -22,8 +22,10 @@
 //   the four taps and three lerps, with no bounds tests (the
 //   texture-address unit, not the ALU, limits a gather kernel);
 // * lanes of a wavefront are a 16 x 4 (x, y) patch, which projects onto a
-//   compact detector patch; workgroups are ordered z-chunk -> y -> x and
-//   grouped per XCD so each L2 sees a contiguous part of the (x, y) plane.
+//   compact detector patch; workgroups go to the XCDs in groups of
+//   4 x 4 x 4 neighbouring tiles, whose detector patches overlap.
+#include <algorithm>
+
 #include "kk_internal.h"
 
 namespace {
@@ -41,6 +43,7 @@ struct BpArgs {
   int32_t mode;  // 0: out = b, 1: out += b, 2: SIRT update
   int32_t tiles_x, tiles_y, tiles_z;
   int32_t xcd_group;
+  int32_t gx, gy, gz;  // tiles per workgroup group along x, y, z
   int64_t total_blocks;
   // mode 2: x <- clip(mask * (x - C * b)), mirrored into the FP textures
   const float* C;
@@ -110,10 +113,21 @@ __device__ __forceinline__ bool run_inside(int32_t base, float lo,
 template <int K, bool TABLE_IN_MEMORY>
 __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
                                         const BpArgs& A, int64_t bid) {
-  const int32_t tx = static_cast<int32_t>(bid % A.tiles_x);
-  const int64_t t = bid / A.tiles_x;
-  const int32_t ty = static_cast<int32_t>(t % A.tiles_y);
-  const int32_t tz = static_cast<int32_t>(t / A.tiles_y);
+  // groups of gx x gy x gz tiles (x fastest inside), groups ordered
+  // x -> y -> z; ids past the tiles pad the grid to whole groups
+  const int32_t gx = A.gx, gy = A.gy, gz = A.gz;
+  const int64_t inner = bid % (gx * gy * gz);
+  const int64_t outer = bid / (gx * gy * gz);
+  const int32_t ngx = (A.tiles_x + gx - 1) / gx;
+  const int32_t ngy = (A.tiles_y + gy - 1) / gy;
+  const int32_t tx = static_cast<int32_t>(outer % ngx) * gx +
+                     static_cast<int32_t>(inner % gx);
+  const int64_t o2 = outer / ngx;
+  const int32_t ty = static_cast<int32_t>(o2 % ngy) * gy +
+                     static_cast<int32_t>((inner / gx) % gy);
+  const int32_t tz = static_cast<int32_t>(o2 / ngy) * gz +
+                     static_cast<int32_t>(inner / (gx * gy));
+  if (tx >= A.tiles_x || ty >= A.tiles_y || tz >= A.tiles_z) return;
   const int32_t ix = tx * 16 + int32_t(threadIdx.x & 15);
   const int32_t iy = ty * 16 + int32_t(threadIdx.x >> 4);
   const int32_t iz0 = tz * K;
@@ -467,14 +481,21 @@ int make_bp_args(const kk::BpSpec& S, int32_t zrun, BpArgs& A) {
   A.tiles_x = (A.nx + 15) / 16;
   A.tiles_y = (A.ny + 15) / 16;
   A.tiles_z = (A.nz + zrun - 1) / zrun;
-  A.total_blocks = int64_t(A.tiles_x) * A.tiles_y * A.tiles_z;
+  // groups of 4 x 4 x 4 tiles (clipped to the tile counts), each group's
+  // ids on one XCD: neighbouring tiles project onto overlapping detector
+  // patches, which that XCD's L2 then serves (measured at 512^3 / 720
+  // angles: 51.5 ms, against 53.3 ms for runs of 16 tiles along x, which
+  // were 1.5% faster than round-robin; 4 x 4 x 8, 8 x 4 x 4, 4 x 4 x 2
+  // and 4 x 4 x 16 took 52.2, 51.7, 52.1 and 52.6 ms)
+  A.gx = std::min(4, A.tiles_x);
+  A.gy = std::min(4, A.tiles_y);
+  A.gz = std::min(4, A.tiles_z);
+  A.xcd_group = A.gx * A.gy * A.gz;
+  A.total_blocks = int64_t((A.tiles_x + A.gx - 1) / A.gx) * A.gx *
+                   ((A.tiles_y + A.gy - 1) / A.gy) * A.gy *
+                   ((A.tiles_z + A.gz - 1) / A.gz) * A.gz;
   if (A.total_blocks >= (int64_t(1) << 31))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: grid too large");
-  const int32_t plane = A.tiles_x * A.tiles_y;
-  // runs of 16 consecutive tiles per XCD (measured at 512^3 / 720 angles:
-  // 1.5% faster than round-robin, 4% faster than one contiguous part of
-  // the (x, y) plane per XCD)
-  A.xcd_group = 16;
   return KK_OK;
 }
 

@@ -898,3 +898,26 @@ def test_fp_transpose_kernel_weights_at_scale(name, wobble, angles, slices):
             wr = _fp_weight(P, a1, a2).astype(np.float64)
             err = np.abs(wk * wr - w) / wr
             assert err.max() < 5e-6, (name, int(P["colc"]), i, err.max())
+
+
+@pytest.mark.parametrize("tiles", [(32, 32, 64), (6, 5, 3), (1, 1, 2),
+                                   (9, 17, 33)])
+def test_bp_tile_decode_covers_every_tile_once(tiles):
+    """The backprojector's workgroup decode (groups of up to 4 x 4 x 4
+    tiles, x fastest inside; groups x -> y -> z), composed with the XCD
+    remap of one group per run, visits every (x, y, z) tile once; padding
+    ids fall outside."""
+    tx_n, ty_n, tz_n = tiles
+    gx, gy, gz = (min(4, t) for t in tiles)
+    total = (-(-tx_n // gx) * gx) * (-(-ty_n // gy) * gy) * \
+        (-(-tz_n // gz) * gz)
+    bid = _xcd_remap(np.arange(total, dtype=np.int64), total, gx * gy * gz)
+    inner, outer = bid % (gx * gy * gz), bid // (gx * gy * gz)
+    ngx, ngy = -(-tx_n // gx), -(-ty_n // gy)
+    tx = (outer % ngx) * gx + inner % gx
+    o2 = outer // ngx
+    ty = (o2 % ngy) * gy + (inner // gx) % gy
+    tz = (o2 // ngy) * gz + inner // (gx * gy)
+    live = (tx < tx_n) & (ty < ty_n) & (tz < tz_n)
+    key = (tz[live] * ty_n + ty[live]) * tx_n + tx[live]
+    assert np.array_equal(np.sort(key), np.arange(tx_n * ty_n * tz_n))
