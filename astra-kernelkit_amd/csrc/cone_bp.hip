@@ -22,8 +22,9 @@
 //   the four taps and three lerps, with no bounds tests (the
 //   texture-address unit, not the ALU, limits a gather kernel);
 // * lanes of a wavefront are a 16 x 4 (x, y) patch, which projects onto a
-//   compact detector patch; workgroups go to the XCDs in groups of
-//   4 x 4 x 4 neighbouring tiles, whose detector patches overlap.
+//   compact detector patch; on circular orbits workgroups go to the XCDs
+//   in groups of 4 x 4 x 4 neighbouring tiles, whose detector patches
+//   overlap, otherwise in runs of 16 tiles along x.
 #include <algorithm>
 
 #include "kk_internal.h"
@@ -114,7 +115,8 @@ template <int K, bool TABLE_IN_MEMORY>
 __device__ __forceinline__ void bp_tile(const kk::BpAngle* __restrict__ tab,
                                         const BpArgs& A, int64_t bid) {
   // groups of gx x gy x gz tiles (x fastest inside), groups ordered
-  // x -> y -> z; ids past the tiles pad the grid to whole groups
+  // x -> y -> z (the group sides divide the tile counts; 1 x 1 x 1 is
+  // plain x -> y -> z order)
   const int32_t gx = A.gx, gy = A.gy, gz = A.gz;
   const int64_t inner = bid % (gx * gy * gz);
   const int64_t outer = bid / (gx * gy * gz);
@@ -481,19 +483,20 @@ int make_bp_args(const kk::BpSpec& S, int32_t zrun, BpArgs& A) {
   A.tiles_x = (A.nx + 15) / 16;
   A.tiles_y = (A.ny + 15) / 16;
   A.tiles_z = (A.nz + zrun - 1) / zrun;
-  // groups of 4 x 4 x 4 tiles (clipped to the tile counts), each group's
-  // ids on one XCD: neighbouring tiles project onto overlapping detector
-  // patches, which that XCD's L2 then serves (measured at 512^3 / 720
-  // angles: 51.5 ms, against 53.3 ms for runs of 16 tiles along x, which
-  // were 1.5% faster than round-robin; 4 x 4 x 8, 8 x 4 x 4, 4 x 4 x 2
-  // and 4 x 4 x 16 took 52.2, 51.7, 52.1 and 52.6 ms)
-  A.gx = std::min(4, A.tiles_x);
-  A.gy = std::min(4, A.tiles_y);
-  A.gz = std::min(4, A.tiles_z);
-  A.xcd_group = A.gx * A.gy * A.gz;
-  A.total_blocks = int64_t((A.tiles_x + A.gx - 1) / A.gx) * A.gx *
-                   ((A.tiles_y + A.gy - 1) / A.gy) * A.gy *
-                   ((A.tiles_z + A.gz - 1) / A.gz) * A.gz;
+  // Circular orbits, tile counts divisible by 4: groups of 4 x 4 x 4
+  // tiles, each group's ids on one XCD; neighbouring tiles project onto
+  // overlapping detector patches, which that XCD's L2 then serves
+  // (measured at 512^3 / 720 angles: 51.5 ms, against 53.3 ms for runs of
+  // 16 tiles along x; 4 x 4 x 8, 8 x 4 x 4, 4 x 4 x 2 and 4 x 4 x 16 took
+  // 52.2, 51.7, 52.1 and 52.6 ms).  Other geometries keep runs of 16
+  // tiles along x per XCD (1.5% faster than round-robin at 512^3): the
+  // groups made the tilted-detector batches of the batching workload
+  // slower (its batched backprojection 57.1 -> 65.8 ms).
+  const bool groups = S.plan->bp_all_circ && A.tiles_x % 4 == 0 &&
+                      A.tiles_y % 4 == 0 && A.tiles_z % 4 == 0;
+  A.gx = A.gy = A.gz = groups ? 4 : 1;
+  A.xcd_group = groups ? 64 : 16;
+  A.total_blocks = int64_t(A.tiles_x) * A.tiles_y * A.tiles_z;
   if (A.total_blocks >= (int64_t(1) << 31))
     return fail(KK_ERR_UNSUPPORTED, "kk_cone_bp: grid too large");
   return KK_OK;

@@ -212,6 +212,7 @@ struct kk_plan_s {
   kk::BpAngle* bp = nullptr;
   int32_t bp_n = 0, bp_cap = 0;
   bool bp_set = false;
+  bool bp_all_circ = false;  // every angle on the circular path (fast == 1)
 };
 
 // Map a hardware workgroup id to a logical tile id such that, inside every

@@ -319,6 +319,9 @@ extern "C" int kk_set_params_bp(kk_plan* plan, const double* params,
   hipSetDevice(dev);
   if (rc != KK_OK) return rc;
   plan->bp_n = n;
+  plan->bp_all_circ = true;
+  for (int32_t k = 0; k < n; ++k)
+    plan->bp_all_circ = plan->bp_all_circ && tab[k].fast == 1;
   plan->bp_set = true;
   return KK_OK;
 }
