@@ -205,7 +205,10 @@ int kk_cone_bp(const kk_plan* plan, const float* proj_tex, int32_t n_angles,
  * 0: vol += A^T proj, else vol = A^T proj.  work: caller-owned device
  * scratch of kk_cone_fp_transpose_work_elems(plan, rows, cols) floats
  * (overwritten; the weighted projections in column-major order with zero
- * rows around each column).  rows * cols < 2^31. */
+ * rows around each column).  rows * cols < 2^31 and
+ * cols * (rows + 32) < 2^30 (one angle's workspace below 4 GiB), else
+ * KK_ERR_UNSUPPORTED.  The angles of each ray axis run in launches of as
+ * many angles as have 192 MiB of workspace, the later ones adding to vol. */
 int kk_cone_fp_transpose(const kk_plan* plan, const float* proj,
                          const int64_t proj_strides[3], int32_t rows,
                          int32_t cols, float* vol, const int32_t vol_shape[3],
