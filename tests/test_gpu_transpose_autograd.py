@@ -202,3 +202,31 @@ def test_unrolled_checkpoint_and_adjoint(weighting, device):
     lhs = float((My.double() * w.double()).sum())
     rhs = float((y1.double() * ya.grad.double()).sum())
     assert lhs == pytest.approx(rhs, rel=1e-4)
+
+
+def test_transpose_rejects_a_workspace_angle_of_4_gib(device):
+    """The gather addresses one angle's padded workspace with 32-bit byte
+    offsets: a 30000 x 40000 detector (fewer than 2^31 pixels, but
+    cols * (rows + 32) * 4 bytes >= 2^32) is refused with
+    KK_ERR_UNSUPPORTED before anything is launched."""
+    import torch
+    from kernelkit import _native
+    from kernelkit.kernels.cone_fp import detector_shape
+    pgs = workloads.circular_geometry(4, sod=150.0, odd=80.0, rows=20,
+                                      cols=24, pixel=1.1, seed=3)
+    vg = kk.resolve_volume_geometry(shape=(12, 14, 10), voxel_size=1.0)
+    op = kk.XrayTransform(pgs, vg)
+    K = op.projector.kernel
+    rows, cols = detector_shape(op.projector._proj_geom_seq)
+    K._upload(rows, cols, vg)
+    y = torch.zeros(16, device="cuda")
+    vol = torch.zeros(vg.shape, device="cuda")
+    work = torch.zeros(16, device="cuda")
+    lib = _native.lib()
+    with pytest.raises(NotImplementedError, match="padded pixels"):
+        _native.check(lib.kk_cone_fp_transpose(
+            K._plan.ptr, _native._vp(y.data_ptr()), _native.i64((1, 1, 1)),
+            30000, 40000, _native._vp(vol.data_ptr()),
+            _native.i32(vg.shape), _native.i64(vol.stride()), 0,
+            _native._vp(work.data_ptr()), _native.stream_ptr()))
+    assert 40000 * (30000 + 32) * 4 >= 2 ** 32 > 30000 * 40000
