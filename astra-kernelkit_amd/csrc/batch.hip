@@ -1,5 +1,5 @@
-// Batches of independent cone-beam problems: one persistent launch per
-// operator for all of them.
+// Batches of independent cone-beam problems: one launch per operator for
+// all of them.
 //
 // The reference handles one geometry per projector (projector.py:160-183)
 // and its autograd operator loops over samples in Python
@@ -9,8 +9,10 @@
 // descriptions (the arguments of kk_cone_fp / kk_cone_bp /
 // kk_sirt_fp_residual / kk_sirt_bp_update), builds device descriptors once
 // for a chosen tile shape, and launches every forward (or backward) problem
-// in one persistent grid that walks the problems' concatenated workgroup
-// tiles.  Texture builds (SIRT residuals into backprojection textures,
+// in one grid that walks the problems' concatenated workgroup tiles with a
+// grid-stride loop: one workgroup per tile, or a persistent grid of a few
+// workgroups per CU (the tuner times both; on the batching workload one
+// per tile wins).  Texture builds (SIRT residuals into backprojection textures,
 // updated volumes into forward-projection frames) are batched the same way.
 // Launches allocate nothing and can be captured in a HIP graph (a SIRT
 // iteration of the whole batch is four launches: forward projections,

@@ -5,8 +5,8 @@ by the caller, ``kernels/cone_fp.py:57-66``, ``kernels/cone_bp.py:52-66``).
 For a batch of problems the tuner times the candidate tile shapes of the
 gfx950 kernels on the batch itself -- rows per thread of the forward
 projector (1, 2, 4), z-run of the backprojector (4, 8, 16) and the size of
-the persistent grid (one workgroup per tile, or a fixed number of
-workgroups per CU) -- with HIP events on the launching stream, and keeps
+the grid (one workgroup per tile, or a persistent grid of a fixed number
+of workgroups per CU) -- with HIP events on the launching stream, and keeps
 the fastest.  Results are cached per batch signature (problem shapes), in
 memory and optionally in the JSON file named by ``KK_TUNE_CACHE``.
 """
@@ -21,11 +21,12 @@ from kernelkit import _native
 
 FP_ROWS = (1, 2, 4)
 BP_ZRUN = (4, 8, 16)
-GRIDS_PER_CU = (0, 4, 8)   # persistent workgroups per CU (0: one per tile)
+GRIDS_PER_CU = (0, 4, 8)   # workgroups per CU (0: one per tile)
 
 
 def _grids():
-    """Candidate persistent grid sizes for the current device."""
+    """Candidate grid sizes for the current device (0: one workgroup per
+    tile, else a persistent grid)."""
     n_cu, _ = _native.device_info()
     return tuple(g * n_cu for g in GRIDS_PER_CU)
 
@@ -33,7 +34,7 @@ def _grids():
 @dataclass(frozen=True)
 class LaunchConfig:
     tile: int      # rows per thread (FP) or z-run (BP)
-    grid: int      # persistent workgroups (0: one per tile)
+    grid: int      # workgroups (0: one per tile, else persistent)
     ms: float = float("nan")
 
 

@@ -5,9 +5,10 @@ No counterpart in the reference: it runs one geometry per projector
 (``torch_support.py:19-20``).  ``ConeBatch`` keeps one ``XrayTransform`` per
 problem (volumes, detectors, angle counts and geometries may all differ) and
 runs all forward projections, all backprojections or a whole SIRT iteration
-of the batch as single persistent launches of the gfx950 kernels
-(``kk_batch_*``).  The tile shapes (rows per thread of the forward kernel,
-z-run of the backprojector, persistent grid size) come from
+of the batch as single launches of the gfx950 kernels (``kk_batch_*``,
+a grid-stride loop over all problems' tiles).  The tile shapes (rows per
+thread of the forward kernel, z-run of the backprojector) and the grid
+size (one workgroup per tile, or a persistent grid) come from
 :class:`~kernelkit.autotune.BatchTuner`.  ``sirt`` can capture its iteration
 loop in a HIP graph.
 """

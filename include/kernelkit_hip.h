@@ -249,7 +249,8 @@ int64_t kk_cone_fp_transpose_static_capacity(void);
  * kk_batch_configure builds device descriptors for one tile shape
  * (fp_rows_per_thread 1/2/4 or 0 = automatic; bp_zrun 4/8/16), and
  * kk_batch_fp / kk_batch_bp run all forward / backward problems in one
- * persistent launch of grid_blocks workgroups (0 = one per tile).  Launches
+ * launch of grid_blocks workgroups walking all tiles in a grid-stride loop
+ * (0 = one workgroup per tile; fewer make the grid persistent).  Launches
  * allocate nothing and are graph-capturable; a SIRT iteration of the batch
  * is kk_batch_fp, kk_batch_residual, kk_batch_bp, kk_batch_volume_textures.
  * Adding a problem drops the descriptors (configure again). */
