@@ -70,6 +70,23 @@ def test_bp_matches_oracle(case, device):
     assert_parity(got, ref, label=f"bp/{case}")
 
 
+@pytest.mark.parametrize("shape", [(64, 64, 32), (64, 60, 40)])
+def test_bp_workgroup_groups_match_oracle(shape, device):
+    """On a circular orbit whose tile counts (16 x 16 columns, z-runs of 8)
+    divide by 4 the backprojector deals its workgroups to the XCDs in
+    groups of 4 x 4 x 4 tiles (64^3-class volumes here: 4 x 4 x 4 tiles);
+    otherwise (64 x 60 x 40: 5 z tiles) in runs along x.  Both orders give
+    the oracle's backprojection; the FP of the same case, 20 angles in
+    groups of 16 (padding ids), does too."""
+    pgs, vg, vol = _small_case(n_angles=20, shape=shape, rows=72, cols=80)
+    proj = oracle.cone_fp(vol, pgs, vg)
+    A = kk.XrayTransform(pgs, vg)
+    assert_parity(_host(A(_dev(vol))), proj, label=f"fp/groups {shape}")
+    ref = oracle.cone_bp(proj, pgs, vg)
+    got = _host(A.T(_dev(proj)))
+    assert_parity(got, ref, label=f"bp/groups {shape}")
+
+
 def test_bp_noise_input(device):
     pgs, vg, _ = _small_case(n_angles=16, seed=5)
     rng = np.random.default_rng(7)
