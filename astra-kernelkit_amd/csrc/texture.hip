@@ -105,20 +105,29 @@ __global__ __launch_bounds__(256) void tex_jobs_kernel(
                        kk::kHaloLo;
     const int32_t r1 = r0 + 1;
     const bool ok0 = r0 >= 0 && r0 < J.n1, ok1 = r1 >= 0 && r1 < J.n1;
-    auto value = [&](int32_t r, int32_t c) {
-      float v = J.src[i * J.s0 + r * J.s1 + c * J.s2];
-      if (J.y != nullptr)
-        v = (v - J.y[i * J.y0 + r * J.y1 + c * J.y2]) *
-            J.R[i * J.r0 + r * J.r1 + c * J.r2];
-      return v;
-    };
+    const bool res = J.y != nullptr;
+    // the two source rows' element offsets, once per texture row (rows
+    // outside the image are never read); per column one multiply per array
+    const int64_t so0 = i * J.s0 + int64_t(ok0 ? r0 : 0) * J.s1;
+    const int64_t so1 = i * J.s0 + int64_t(ok1 ? r1 : 0) * J.s1;
+    const int64_t yo0 = res ? i * J.y0 + int64_t(ok0 ? r0 : 0) * J.y1 : 0;
+    const int64_t yo1 = res ? i * J.y0 + int64_t(ok1 ? r1 : 0) * J.y1 : 0;
+    const int64_t ro0 = res ? i * J.r0 + int64_t(ok0 ? r0 : 0) * J.r1 : 0;
+    const int64_t ro1 = res ? i * J.r0 + int64_t(ok1 ? r1 : 0) * J.r1 : 0;
     float2* out = J.tex + lr * tc;
     for (int32_t ju = threadIdx.x; ju < tc; ju += 256) {
       const int32_t c = ju - kk::kHaloLo;
       float v0 = 0.f, v1 = 0.f;
       if (c >= 0 && c < J.n2) {
-        if (ok0) v0 = value(r0, c);
-        if (ok1) v1 = value(r1, c);
+        const int64_t sc = c * J.s2;
+        if (res) {
+          const int64_t yc = c * J.y2, rc = c * J.r2;
+          if (ok0) v0 = (J.src[so0 + sc] - J.y[yo0 + yc]) * J.R[ro0 + rc];
+          if (ok1) v1 = (J.src[so1 + sc] - J.y[yo1 + yc]) * J.R[ro1 + rc];
+        } else {
+          if (ok0) v0 = J.src[so0 + sc];
+          if (ok1) v1 = J.src[so1 + sc];
+        }
       }
       out[ju] = make_float2(v0, v1);
     }
