@@ -15,6 +15,12 @@
 
 namespace {
 
+// Texture rows per workgroup of the two builders below: a thread walks one
+// texture column down kTexRows rows, carrying the previous source row's
+// value (each source element is read once; a row's texels need rows r - 1
+// and r).
+constexpr int32_t kTexRows = 16;
+
 // Frame texture of ray axis a: element (i, jb, jc) of a [na][nb+3][nc+3]
 // float2 array = (V(i, jb-1, jc-1), V(i, jb, jc-1)), zero outside.
 __global__ __launch_bounds__(256) void fp_frame_kernel(
@@ -23,18 +29,23 @@ __global__ __launch_bounds__(256) void fp_frame_kernel(
   const int32_t tc = nc + kk::kHalo;
   const int32_t tb = nb + kk::kHalo;
   const int32_t jc = blockIdx.x * 256 + threadIdx.x;
-  const int32_t jb = blockIdx.y;
+  const int32_t jb0 = blockIdx.y * kTexRows;
   const int32_t i = blockIdx.z;
   if (jc >= tc) return;
   const int32_t c = jc - kk::kHaloLo;
-  const int32_t b0 = jb - kk::kHaloLo, b1 = b0 + 1;
-  float v0 = 0.f, v1 = 0.f;
-  if (c >= 0 && c < nc) {
-    const float* p = vol + i * sa + c * sc;
-    if (b0 >= 0 && b0 < nb) v0 = p[b0 * sb];
-    if (b1 >= 0 && b1 < nb) v1 = p[b1 * sb];
+  const bool col = c >= 0 && c < nc;
+  const float* p = vol + i * sa + (col ? c : 0) * sc;
+  auto value = [&](int32_t b) {
+    return col && b >= 0 && b < nb ? p[b * sb] : 0.f;
+  };
+  const int32_t jb1 = min(jb0 + kTexRows, tb);
+  float prev = value(jb0 - kk::kHaloLo);
+  float2* out = tex + (int64_t(i) * tb + jb0) * tc + jc;
+  for (int32_t jb = jb0; jb < jb1; ++jb, out += tc) {
+    const float cur = value(jb - kk::kHaloLo + 1);
+    *out = make_float2(prev, cur);
+    prev = cur;
   }
-  tex[(int64_t(i) * tb + jb) * tc + jc] = make_float2(v0, v1);
 }
 
 // Projection texture: element (a, jv, ju) of a [n][rows+3][cols+3] float2
@@ -55,24 +66,28 @@ __global__ __launch_bounds__(256) void bp_tex_kernel(
   const int32_t tc = cols + kk::kHalo;
   const int32_t tr = rows + kk::kHalo;
   const int32_t ju = blockIdx.x * 256 + threadIdx.x;
-  const int32_t jv = blockIdx.y;
+  const int32_t jv0 = blockIdx.y * kTexRows;
   const int32_t a = blockIdx.z;
   if (ju >= tc) return;
   const int32_t c = ju - kk::kHaloLo;
-  const int32_t r0 = jv - kk::kHaloLo, r1 = r0 + 1;
+  const bool col = c >= 0 && c < cols;
+  const int32_t cc = col ? c : 0;
   auto value = [&](int32_t r) {
-    float v = proj[a * s0 + r * s1 + c * s2];
+    if (!col || r < 0 || r >= rows) return 0.f;
+    float v = proj[a * s0 + r * s1 + cc * s2];
     if constexpr (RESID)
-      v = (v - res.y[a * res.y0 + r * res.y1 + c * res.y2]) *
-          res.R[a * res.r0 + r * res.r1 + c * res.r2];
+      v = (v - res.y[a * res.y0 + r * res.y1 + cc * res.y2]) *
+          res.R[a * res.r0 + r * res.r1 + cc * res.r2];
     return v;
   };
-  float v0 = 0.f, v1 = 0.f;
-  if (c >= 0 && c < cols) {
-    if (r0 >= 0 && r0 < rows) v0 = value(r0);
-    if (r1 >= 0 && r1 < rows) v1 = value(r1);
+  const int32_t jv1 = min(jv0 + kTexRows, tr);
+  float prev = value(jv0 - kk::kHaloLo);
+  float2* out = tex + (int64_t(a) * tr + jv0) * tc + ju;
+  for (int32_t jv = jv0; jv < jv1; ++jv, out += tc) {
+    const float cur = value(jv - kk::kHaloLo + 1);
+    *out = make_float2(prev, cur);
+    prev = cur;
   }
-  tex[(int64_t(a) * tr + jv) * tc + ju] = make_float2(v0, v1);
 }
 
 // Many texture builds in one launch: block b converts the texture rows
@@ -160,7 +175,9 @@ extern "C" int kk_volume_to_fp_texture(const float* vol,
     if (shape[b] + kk::kHalo > 65535 || shape[a] > 65535)
       return kk::fail(KK_ERR_UNSUPPORTED, "kk_volume_to_fp_texture: large");
     const dim3 grid(unsigned((shape[c] + kk::kHalo + 255) / 256),
-                    unsigned(shape[b] + kk::kHalo), unsigned(shape[a]));
+                    unsigned((shape[b] + kk::kHalo + kTexRows - 1) /
+                             kTexRows),
+                    unsigned(shape[a]));
     hipLaunchKernelGGL(fp_frame_kernel, grid, dim3(256), 0,
                        static_cast<hipStream_t>(stream), vol, st[a], st[b],
                        st[c], shape[b], shape[c],
@@ -184,7 +201,8 @@ extern "C" int kk_projections_to_bp_texture(const float* proj, int32_t n,
     return kk::fail(KK_ERR_UNSUPPORTED,
                     "kk_projections_to_bp_texture: too large");
   const dim3 grid(unsigned((cols + kk::kHalo + 255) / 256),
-                  unsigned(rows + kk::kHalo), unsigned(n));
+                  unsigned((rows + kk::kHalo + kTexRows - 1) / kTexRows),
+                  unsigned(n));
   hipLaunchKernelGGL(bp_tex_kernel<false>, grid, dim3(256), 0,
                      static_cast<hipStream_t>(stream), proj, st[0], st[1],
                      st[2], rows, cols, reinterpret_cast<float2*>(tex),
@@ -207,7 +225,8 @@ extern "C" int kk_residual_to_bp_texture(
     return kk::fail(KK_ERR_UNSUPPORTED,
                     "kk_residual_to_bp_texture: too large");
   const dim3 grid(unsigned((cols + kk::kHalo + 255) / 256),
-                  unsigned(rows + kk::kHalo), unsigned(n_angles));
+                  unsigned((rows + kk::kHalo + kTexRows - 1) / kTexRows),
+                  unsigned(n_angles));
   const ResidArgs res{y, ys[0], ys[1], ys[2], R, rs[0], rs[1], rs[2]};
   hipLaunchKernelGGL(bp_tex_kernel<true>, grid, dim3(256), 0,
                      static_cast<hipStream_t>(stream), proj, ps[0], ps[1],
