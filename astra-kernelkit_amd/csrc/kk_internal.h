@@ -152,7 +152,7 @@ struct TexJob {
   int64_t r0 = 0, r1 = 0, r2 = 0;
   float2* tex = nullptr;
   int32_t n0 = 0, n1 = 0, n2 = 0;
-  int64_t first = 0;  // first texture row of this job in the batch
+  int64_t first = 0;  // first tile of this job in the batch
 };
 struct TexJobs;
 int tex_jobs_build(std::vector<TexJob> jobs, TexJobs** out);

@@ -90,61 +90,53 @@ __global__ __launch_bounds__(256) void bp_tex_kernel(
   }
 }
 
-// Many texture builds in one launch: block b converts the texture rows
-// [b chunk, (b + 1) chunk) of the jobs' concatenated rows, finding its
-// first job by binary search over their first rows (wave-uniform) and
-// stepping to the next job when a row passes its end.  Each element is the
-// same arithmetic as fp_frame_kernel / bp_tex_kernel.
+// Many texture builds in one launch: a grid-stride loop over the jobs'
+// tiles (one image's band of kTexRows texture rows by 256 columns, column
+// chunks fastest), each found by binary search over the jobs' first tile
+// ids (wave-uniform).  A thread walks one column of its tile like
+// fp_frame_kernel / bp_tex_kernel, with the same arithmetic per element.
 __global__ __launch_bounds__(256) void tex_jobs_kernel(
-    const kk::TexJob* __restrict__ jobs, int32_t n_jobs, int64_t total,
-    int64_t chunk) {
-  const int64_t t0 = int64_t(blockIdx.x) * chunk;
-  if (t0 >= total) return;
-  const int64_t t1 = t0 + chunk < total ? t0 + chunk : total;
-  int32_t lo = 0, hi = n_jobs - 1;
-  while (lo < hi) {
-    const int32_t mid = (lo + hi + 1) >> 1;
-    if (jobs[mid].first <= t0)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  int32_t j = lo;
-  for (int64_t t = t0; t < t1; ++t) {
-    while (j + 1 < n_jobs && jobs[j + 1].first <= t) ++j;
-    const kk::TexJob& J = jobs[j];
+    const kk::TexJob* __restrict__ jobs, int32_t n_jobs, int64_t total) {
+  for (int64_t g = blockIdx.x; g < total; g += gridDim.x) {
+    int32_t lo = 0, hi = n_jobs - 1;
+    while (lo < hi) {
+      const int32_t mid = (lo + hi + 1) >> 1;
+      if (jobs[mid].first <= g)
+        lo = mid;
+      else
+        hi = mid - 1;
+    }
+    const kk::TexJob& J = jobs[lo];
     const int32_t tr = J.n1 + kk::kHalo, tc = J.n2 + kk::kHalo;
-    const int64_t lr = t - J.first;  // texture row within the job
-    const int32_t i = static_cast<int32_t>(lr / tr);
-    const int32_t r0 = static_cast<int32_t>(lr - int64_t(i) * tr) -
-                       kk::kHaloLo;
-    const int32_t r1 = r0 + 1;
-    const bool ok0 = r0 >= 0 && r0 < J.n1, ok1 = r1 >= 0 && r1 < J.n1;
+    const int32_t nch = (tc + 255) / 256;
+    const int32_t nbands = (tr + kTexRows - 1) / kTexRows;
+    const int64_t local = g - J.first;
+    const int32_t chunk = static_cast<int32_t>(local % nch);
+    const int64_t t = local / nch;
+    const int32_t jv0 = static_cast<int32_t>(t % nbands) * kTexRows;
+    const int32_t i = static_cast<int32_t>(t / nbands);
+    const int32_t ju = chunk * 256 + static_cast<int32_t>(threadIdx.x);
+    if (ju >= tc) continue;
+    const int32_t c = ju - kk::kHaloLo;
+    const bool col = c >= 0 && c < J.n2;
+    const int64_t cc = col ? c : 0;
     const bool res = J.y != nullptr;
-    // the two source rows' element offsets, once per texture row (rows
-    // outside the image are never read); per column one multiply per array
-    const int64_t so0 = i * J.s0 + int64_t(ok0 ? r0 : 0) * J.s1;
-    const int64_t so1 = i * J.s0 + int64_t(ok1 ? r1 : 0) * J.s1;
-    const int64_t yo0 = res ? i * J.y0 + int64_t(ok0 ? r0 : 0) * J.y1 : 0;
-    const int64_t yo1 = res ? i * J.y0 + int64_t(ok1 ? r1 : 0) * J.y1 : 0;
-    const int64_t ro0 = res ? i * J.r0 + int64_t(ok0 ? r0 : 0) * J.r1 : 0;
-    const int64_t ro1 = res ? i * J.r0 + int64_t(ok1 ? r1 : 0) * J.r1 : 0;
-    float2* out = J.tex + lr * tc;
-    for (int32_t ju = threadIdx.x; ju < tc; ju += 256) {
-      const int32_t c = ju - kk::kHaloLo;
-      float v0 = 0.f, v1 = 0.f;
-      if (c >= 0 && c < J.n2) {
-        const int64_t sc = c * J.s2;
-        if (res) {
-          const int64_t yc = c * J.y2, rc = c * J.r2;
-          if (ok0) v0 = (J.src[so0 + sc] - J.y[yo0 + yc]) * J.R[ro0 + rc];
-          if (ok1) v1 = (J.src[so1 + sc] - J.y[yo1 + yc]) * J.R[ro1 + rc];
-        } else {
-          if (ok0) v0 = J.src[so0 + sc];
-          if (ok1) v1 = J.src[so1 + sc];
-        }
-      }
-      out[ju] = make_float2(v0, v1);
+    const float* sp = J.src + i * J.s0 + cc * J.s2;
+    const float* yp = res ? J.y + i * J.y0 + cc * J.y2 : nullptr;
+    const float* rp = res ? J.R + i * J.r0 + cc * J.r2 : nullptr;
+    auto value = [&](int32_t r) {
+      if (!col || r < 0 || r >= J.n1) return 0.f;
+      float v = sp[r * J.s1];
+      if (res) v = (v - yp[r * J.y1]) * rp[r * J.r1];
+      return v;
+    };
+    const int32_t jv1 = min(jv0 + kTexRows, tr);
+    float prev = value(jv0 - kk::kHaloLo);
+    float2* out = J.tex + (int64_t(i) * tr + jv0) * tc + ju;
+    for (int32_t jv = jv0; jv < jv1; ++jv, out += tc) {
+      const float cur = value(jv - kk::kHaloLo + 1);
+      *out = make_float2(prev, cur);
+      prev = cur;
     }
   }
 }
@@ -254,7 +246,7 @@ namespace kk {
 struct TexJobs {
   TexJob* jobs = nullptr;
   int32_t n = 0;
-  int64_t total = 0;  // texture rows of all jobs
+  int64_t total = 0;  // tiles of all jobs (see tex_jobs_kernel)
 };
 
 int tex_jobs_build(std::vector<TexJob> jobs, TexJobs** out) {
@@ -267,7 +259,8 @@ int tex_jobs_build(std::vector<TexJob> jobs, TexJobs** out) {
     if (int64_t(J.n0) * (J.n1 + kHalo) >= (int64_t(1) << 31))
       return fail(KK_ERR_UNSUPPORTED, "texture batch: job too large");
     J.first = total;
-    total += int64_t(J.n0) * (J.n1 + kHalo);
+    total += int64_t(J.n0) * ((J.n1 + kHalo + kTexRows - 1) / kTexRows) *
+             ((J.n2 + kHalo + 255) / 256);
   }
   auto* b = new (std::nothrow) TexJobs();
   if (b == nullptr) return fail(KK_ERR_INVALID, "out of host memory");
@@ -293,14 +286,10 @@ void tex_jobs_free(TexJobs* b) {
 
 int tex_jobs_launch(const TexJobs* b, hipStream_t stream) {
   if (b == nullptr) return fail(KK_ERR_NOT_SET, "texture batch not built");
-  // ~16 rows of a few hundred float2 per workgroup: enough workgroups to
-  // fill the device, few enough to amortise each one's job search
-  int64_t blocks = (b->total + 15) / 16;
-  if (blocks > 65536) blocks = 65536;
-  const int64_t chunk = (b->total + blocks - 1) / blocks;
-  blocks = (b->total + chunk - 1) / chunk;
+  // a tile per workgroup, at most 1024 workgroups per CU of the 256
+  const int64_t blocks = b->total < 262144 ? b->total : 262144;
   hipLaunchKernelGGL(tex_jobs_kernel, dim3(static_cast<unsigned>(blocks)),
-                     dim3(256), 0, stream, b->jobs, b->n, b->total, chunk);
+                     dim3(256), 0, stream, b->jobs, b->n, b->total);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return hip_fail(e, "tex_jobs_kernel");
   return KK_OK;
