@@ -203,3 +203,38 @@ def test_xray_backprojection_module_and_gradient(device):
     for b in range(2):
         assert_parity(_host(yt.grad[b, 0]), oracle.cone_fp(w[b, 0], pgs, vg),
                       label=f"XrayBackprojection grad {b}")
+
+
+def test_library_graph_capture_of_a_projector_call(device):
+    """``_native.Graph.capture`` (the library's own stream capture, no
+    torch.cuda.graph) records ``ptor.projections = y; ptor()`` of a
+    backprojector; three replays with new projections, eager work and a
+    synchronisation between them, each bit-equal to the eager call
+    (``example/cuda/graph.py:46-51`` records the same call through CuPy)."""
+    torch = _torch()
+    from kernelkit import _native
+    pgs, vg = _case(wobble_deg=1.0, drift_px=2.0)
+    rng = np.random.default_rng(4)
+    projs = _dev(rng.uniform(size=(len(pgs), 34, 40)))
+    out = torch.zeros(vg.shape, device="cuda")
+    ptor = kk.BackProjector(texture_type="array")
+    ptor.projection_geometry = pgs
+    ptor.volume_geometry = vg
+    ptor.projections = projs
+    ptor.volume = out
+    ptor()                            # warm-up: tables and texture exist
+    torch.cuda.synchronize()
+
+    def call():
+        ptor.projections = projs
+        ptor()
+    g = _native.Graph.capture(call)
+    eager_op = kk.XrayTransform(pgs, vg)
+    for k in range(3):
+        projs.copy_(_dev(rng.uniform(size=projs.shape)))
+        out.fill_(-1.0)
+        g.replay()
+        torch.cuda.synchronize()
+        ref = eager_op.T(projs)
+        _ = (ref * 2.0).sum().item()  # eager work between replays
+        assert torch.equal(out, ref), f"replay {k}"
