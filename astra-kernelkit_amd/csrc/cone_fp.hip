@@ -67,10 +67,32 @@ constexpr int kColsBatch = 4;
 // problems; kGroupColsBatch in batches) neighbouring column tiles go to
 // one XCD together.  An XCD holds 64 single-problem workgroups (32 CUs x
 // 2 of 1024 threads) or 256 batch workgroups (32 x 8 of 256 threads);
-// measured on MI355X: 16 x 8 and 16 x 16 were the fastest groups.
+// measured on MI355X: 16 x 8 and 16 x 16 were the fastest groups.  These
+// are upper bounds: a group size is the one in [ceil(cap / 2), cap] that
+// pads the angles or column tiles least (group_size).
 constexpr int32_t kGroupAngles = 16;
 constexpr int32_t kGroupCols = 8;
 constexpr int32_t kGroupColsBatch = 16;
+
+// Group size for n items, at most cap: the g in [ceil(cap / 2), cap] whose
+// whole groups pad n least (the larger g on ties), or n if n <= cap.  Padded
+// ids return at once, but the groups go to the XCDs whole, so an XCD dealt
+// mostly padded groups idles while others work (160 detector columns = 10
+// tiles in groups of 8: about a fifth more time at 128^3 / 180 angles than
+// groups of 5).
+int32_t group_size(int32_t n, int32_t cap) {
+  if (n <= cap) return n;
+  int32_t best = cap;
+  int64_t best_pad = -1;
+  for (int32_t g = cap; g >= (cap + 1) / 2; --g) {
+    const int64_t pad = int64_t((n + g - 1) / g) * g - n;
+    if (best_pad < 0 || pad < best_pad) {
+      best = g;
+      best_pad = pad;
+    }
+  }
+  return best;
+}
 
 struct FpArgs {
   const char* tex;  // frame textures (byte base)
@@ -489,10 +511,10 @@ int make_fp_args(const kk::FpSpec& S, int32_t rpt, int32_t cols_per_block,
   A.row_tiles = (S.rows + kWave * rpt - 1) / (kWave * rpt);
   A.col_tiles = (S.cols + cols_per_block - 1) / cols_per_block;
   A.mode = S.mode;
-  A.grp_a = std::min(kGroupAngles, A.n_angles);
-  A.grp_c = std::min(
-      cols_per_block == kColsSingle ? kGroupCols : kGroupColsBatch,
-      A.col_tiles);
+  A.grp_a = group_size(A.n_angles, kGroupAngles);
+  A.grp_c = group_size(
+      A.col_tiles,
+      cols_per_block == kColsSingle ? kGroupCols : kGroupColsBatch);
   A.xcd_group = A.grp_a * A.grp_c;
   A.total_blocks = int64_t(A.row_tiles) *
                    ((A.n_angles + A.grp_a - 1) / A.grp_a) * A.grp_a *

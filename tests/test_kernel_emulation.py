@@ -810,8 +810,62 @@ def _fp_tile_decode(bid, n_ang, col_tiles, ga, gc):
     return ang, ct, t // ncg
 
 
+def _group_size(n, cap):
+    """cone_fp.hip's group_size: the g in [ceil(cap/2), cap] whose whole
+    groups pad n least (the larger g on ties), n itself if n <= cap."""
+    if n <= cap:
+        return n
+    return min(range(cap, (cap + 1) // 2 - 1, -1),
+               key=lambda g: -(-n // g) * g - n)
+
+
+@pytest.mark.parametrize("n,cap,want", [
+    (720, 16, 16), (1440, 16, 16), (360, 16, 15), (180, 16, 15),
+    (181, 16, 14), (17, 16, 9), (7, 16, 7), (48, 8, 8), (32, 8, 8),
+    (10, 8, 5), (9, 8, 5), (40, 16, 10), (3, 8, 3)])
+def test_fp_group_size_pads_least(n, cap, want):
+    """Group sizes of the forward kernel's workgroup order: the least
+    padding over [ceil(cap/2), cap], brute-forced here against the closed
+    rule, and the chosen sizes for the workloads' angle and column-tile
+    counts."""
+    g = _group_size(n, cap)
+    assert g == want
+    if n > cap:
+        pads = {h: -(-n // h) * h - n for h in range((cap + 1) // 2, cap + 1)}
+        best = min(pads.values())
+        assert pads[g] == best and g == max(h for h in pads
+                                            if pads[h] == best)
+
+
+def _xcd_live_tiles(n_ang, col_tiles, row_tiles, ga, gc):
+    """Live (angle, column tile, row band) workgroups per XCD when the
+    hardware deals workgroup ids round-robin to the 8 XCDs."""
+    total = (row_tiles * (-(-n_ang // ga)) * ga *
+             (-(-col_tiles // gc)) * gc)
+    hw = np.arange(total, dtype=np.int64)
+    ang, ct, _ = _fp_tile_decode(_xcd_remap(hw, total, ga * gc), n_ang,
+                                 col_tiles, ga, gc)
+    live = (ang < n_ang) & (ct < col_tiles)
+    return np.bincount(hw[live] % 8, minlength=8)
+
+
+def test_fp_groups_balance_the_xcds_at_the_small_workload():
+    """128^3 / 180 angles / 160^2 detector: 10 column tiles of 16 and one
+    row band.  Groups of 16 x 8 pad to 192 angles and 16 column tiles and
+    deal the XCDs 168 to 288 live workgroups (mean 225: the busiest XCD
+    has 28% more than its share); the least-padding groups (15 x 5) deal
+    every XCD 225."""
+    old = _xcd_live_tiles(180, 10, 1, 16, 8)
+    assert (old.min(), old.max(), old.sum()) == (168, 288, 1800)
+    ga, gc = _group_size(180, 16), _group_size(10, 8)
+    assert (ga, gc) == (15, 5)
+    new = _xcd_live_tiles(180, 10, 1, ga, gc)
+    assert new.sum() == 1800 and new.min() == new.max()
+
+
 @pytest.mark.parametrize("n_ang,col_tiles,row_tiles,ga,gc", [
     (720, 48, 6, 16, 8),      # 512^3 / 720 angles, 768^2: no padding
+    (180, 10, 1, 15, 5),      # 128^3 / 180, 160^2: least-padding groups
     (360, 24, 3, 16, 8),      # 256^3 / 360, 384^2
     (45, 10, 2, 16, 8),       # padding along angles and column tiles
     (7, 3, 1, 7, 3),          # groups clipped to the problem
