@@ -85,19 +85,30 @@ def preweight_params(geoms, detector_piercings=None) -> np.ndarray:
     ``u * pw``, ``v * ph`` and the central ray length ``|p - s|`` where ``p``
     is the detector centre or the given piercing point."""
     geoms = list(geoms)
-    out = np.empty((len(geoms), 10))
-    for i, g in enumerate(geoms):
-        s = np.asarray(g.source_position, dtype=np.float64)
-        d = np.asarray(g.detector_position, dtype=np.float64)
-        U = np.asarray(g.u, dtype=np.float64) * float(g.detector.pixel_width)
-        V = np.asarray(g.v, dtype=np.float64) * float(g.detector.pixel_height)
-        dmin = d - V * (g.detector.rows / 2) - U * (g.detector.cols / 2)
-        p = d if detector_piercings is None else np.asarray(
-            detector_piercings[i], dtype=np.float64)
-        out[i, 0:3] = dmin - s
-        out[i, 3:6] = U
-        out[i, 6:9] = V
-        out[i, 9] = np.linalg.norm(p - s)
+    n = len(geoms)
+
+    def rows3(attr):
+        return np.array([getattr(g, attr) for g in geoms],
+                        dtype=np.float64).reshape(n, 3)
+
+    def per_angle(get):
+        return np.array([get(g.detector) for g in geoms],
+                        dtype=np.float64).reshape(n, 1)
+
+    s, d = rows3("source_position"), rows3("detector_position")
+    U = rows3("u") * per_angle(lambda det: det.pixel_width)
+    V = rows3("v") * per_angle(lambda det: det.pixel_height)
+    dmin = (d - V * (per_angle(lambda det: det.rows) / 2)
+            - U * (per_angle(lambda det: det.cols) / 2))
+    p = d if detector_piercings is None else np.asarray(
+        detector_piercings, dtype=np.float64).reshape(n, 3)
+    out = np.empty((n, 10))
+    out[:, 0:3] = dmin - s
+    out[:, 3:6] = U
+    out[:, 6:9] = V
+    ps = p - s
+    out[:, 9] = np.sqrt(ps[:, 0] * ps[:, 0] + ps[:, 1] * ps[:, 1]
+                        + ps[:, 2] * ps[:, 2])
     return out
 
 
