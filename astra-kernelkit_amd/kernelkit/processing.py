@@ -12,6 +12,7 @@ follows, ``processing.py:49-50``).
 """
 from __future__ import annotations
 
+from collections import OrderedDict
 from functools import lru_cache
 
 import numpy as np
@@ -161,6 +162,37 @@ def filter(projections, verbose: bool = False,
         writeback()
 
 
+_TABLES: "OrderedDict[tuple, _native.Filter]" = OrderedDict()
+_TABLES_KEPT = 8
+
+
+def _filter_tables(num: int, filter: str, P) -> "_native.Filter":
+    """Device tables (twiddles, response, pre-weights ``P`` or none) of a
+    fused filter call, kept for the last few (device, length, filter,
+    pre-weight) combinations.  A table set is written once, when made, and
+    never changed: a later call with the same key launches on it while
+    earlier launches may still read it, with no upload, allocation or free
+    (whose device-wide wait would otherwise end every call)."""
+    from kernelkit import device
+    key = (device.current_device(), num, str(filter).lower(),
+           None if P is None else (P.shape, P.tobytes()))
+    f = _TABLES.get(key)
+    if f is not None:
+        _TABLES.move_to_end(key)
+        return f
+    f = _native.Filter()
+    H = np.ascontiguousarray(_filter(num, filter), dtype=np.float64)
+    _native.check(_native.lib().kk_filter_set_response(
+        f.ptr, _native.f64_ptr(H), num))
+    if P is not None:
+        _native.check(_native.lib().kk_filter_set_preweight(
+            f.ptr, _native.f64_ptr(P), P.shape[0]))
+    _TABLES[key] = f
+    while len(_TABLES) > _TABLES_KEPT:
+        _TABLES.popitem(last=False)
+    return f
+
+
 def filter_into(projections, texture, geoms=None, filter: str = "ramlak",
                 detector_piercings=None) -> None:
     """Fused FDK pre-processing into a backprojection texture:
@@ -176,15 +208,11 @@ def filter_into(projections, texture, geoms=None, filter: str = "ramlak",
         raise ValueError(f"texture of shape {texture.shape} does not match "
                          f"projections of shape {(n, rows, cols)}")
     num = padded_length(cols)
-    # a private table set: the pre-weights are specific to this call
-    f = _native.Filter()
-    H = np.ascontiguousarray(_filter(num, filter), dtype=np.float64)
-    _native.check(_native.lib().kk_filter_set_response(
-        f.ptr, _native.f64_ptr(H), num))
-    if geoms is not None:
-        P = np.ascontiguousarray(preweight_params(geoms, detector_piercings))
-        _native.check(_native.lib().kk_filter_set_preweight(
-            f.ptr, _native.f64_ptr(P), n))
+    P = None if geoms is None else np.ascontiguousarray(
+        preweight_params(geoms, detector_piercings))
+    if P is not None and P.shape[0] != n:
+        raise ValueError(f"{P.shape[0]} geometries for {n} projections")
+    f = _filter_tables(num, filter, P)
     _native.check(_native.lib().kk_filter_rows_to_bp_texture(
         f.ptr, _native._vp(projections.data_ptr()),
         _native.i64(projections.stride()), _native._vp(texture.ptr), n, rows,

@@ -195,3 +195,46 @@ def test_fused_residual_equals_two_step(pert, device):
     assert float(np.abs(ref).max()) > 0
     assert torch.equal(texs[0].data, texs[1].data)
     assert torch.equal(texs[2].data, texs[1].data)
+
+
+@pytest.mark.gpu
+def test_fused_filter_tables_are_reused_and_keyed_by_preweight():
+    """``filter_into`` keeps its device tables per (device, length, filter,
+    pre-weight table): a repeated call reuses them and writes the same
+    texture bit for bit; another geometry or filter gets tables of its own
+    and matches a fresh call; a geometry count that differs from the
+    projection count raises before any launch."""
+    import torch
+    from kernelkit import processing
+    from kernelkit.kernel import ProjectionTexture
+    pgs = workloads.circular_geometry(12, sod=200.0, odd=100.0, rows=20,
+                                      cols=30, pixel=1.0, seed=1)
+    # another source-to-detector distance (the cosine weights depend only
+    # on it and the pixel positions)
+    pgs2 = workloads.circular_geometry(12, sod=210.0, odd=110.0, rows=20,
+                                       cols=30, pixel=1.0, seed=2)
+    y = torch.from_numpy(np.random.default_rng(5).uniform(
+        size=(12, 20, 30)).astype(np.float32)).cuda()
+
+    def run(geoms, name):
+        tex = ProjectionTexture(*y.shape, like=y)
+        tex.zero()
+        processing.filter_into(y, tex, geoms=geoms, filter=name)
+        torch.cuda.synchronize()
+        return tex
+
+    processing._TABLES.clear()
+    a = run(pgs, "ramlak")
+    assert len(processing._TABLES) == 1
+    b = run(pgs, "ramlak")
+    assert len(processing._TABLES) == 1
+    assert torch.equal(a.data, b.data)
+    c = run(pgs2, "ramlak")
+    d = run(pgs, "hann")
+    assert len(processing._TABLES) == 3
+    processing._TABLES.clear()
+    assert torch.equal(run(pgs2, "ramlak").data, c.data)
+    assert torch.equal(run(pgs, "hann").data, d.data)
+    assert not torch.equal(a.data, c.data) and not torch.equal(a.data, d.data)
+    with pytest.raises(ValueError, match="11 geometries for 12"):
+        run(pgs[:11], "ramlak")
